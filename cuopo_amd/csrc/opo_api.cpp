@@ -1,0 +1,885 @@
+// opo_api.cpp -- host driver and C ABI of libcuopo_amd.so.
+//
+// An Engine owns one HIP stream and a set of Groups; a Group is every
+// instance that shares the grid size N (so one launch of each stage covers the
+// whole group).  opo_solver is an Engine with one instance, opo_batch one
+// with many.  The round trip of src/cuOPO.cu:474-610 is emitted as
+//   [row LIN, col MID] x (NZ-1), row LIN, col BOUNDARY, tick
+// (plus col A / row GDD / col B when GDD compensation is on) and captured once
+// into a hipGraph that is replayed for every steady round trip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/opo.h"
+#include "opo_internal.h"
+
+using opo::OpoInst;
+using cd = std::complex<double>;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                           \
+  do {                                                                                      \
+    hipError_t e_ = (x);                                                                    \
+    if (e_ != hipSuccess) return fail(OPO_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define RCHK(x)                    \
+  do {                             \
+    int r_ = (x);                  \
+    if (r_ != OPO_OK) return r_;   \
+  } while (0)
+
+constexpr long double kPiL = 3.141592653589793238462643383279502884L;
+constexpr double kPi = 3.14159265358979323846;
+
+int ilog2ll(long long x) {
+  int l = 0;
+  while ((1LL << l) < x) ++l;
+  return l;
+}
+bool is_pow2(long long x) { return x > 0 && (x & (x - 1)) == 0; }
+
+// e^{2 pi i e / M} for e = e0 + j*step, j < count, in long double then rounded
+std::vector<cd> roots(long long count, long long step, long long M) {
+  std::vector<cd> r((size_t)count);
+  for (long long j = 0; j < count; ++j) {
+    const long long e = (j * step) % M;
+    const long double a = 2.0L * kPiL * (long double)e / (long double)M;
+    r[(size_t)j] = cd((double)cosl(a), (double)sinl(a));
+  }
+  return r;
+}
+
+struct DevMem {
+  void *p = nullptr;
+  ~DevMem() { if (p) (void)hipFree(p); }
+};
+
+struct Instance {
+  opo_params prm{};
+  void *state = nullptr, *lin = nullptr, *gdd = nullptr, *eom = nullptr, *pump = nullptr, *snap = nullptr;
+  void *save[3] = {nullptr, nullptr, nullptr};
+  double *nl = nullptr;
+  long long pump_len = 0;
+  bool uploaded = false;
+};
+
+struct Group {
+  int N = 0, N1 = 0, N2 = 0, NF = 0, fp32 = 0, lobits = 0;
+  std::vector<int> members;
+  std::vector<Instance> inst;
+  void *tw1 = nullptr, *tw2 = nullptr, *twlo = nullptr, *twhi = nullptr;
+  OpoInst *d_insts = nullptr;
+  int *d_rt = nullptr, *d_active = nullptr, *d_conv = nullptr, *d_status = nullptr;
+  double *d_partials = nullptr, *d_pref = nullptr, *d_hist = nullptr;
+  int hist_len = 0;
+  int ct = 0, rt = 0;  // column tile (columns per block), row tile (rows per block)
+  bool any_gdd = false;
+  std::vector<void *> allocs;
+};
+
+struct Engine {
+  std::vector<Group> groups;
+  int count = 0, NZ = 0, three_eqs = 0, regime = 0, fp32 = 0, device = 0, NRT = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool use_graph = true;
+  bool save_enabled = false;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  int graph_key = -1;
+  float last_ms = 0;
+  int last_rt0 = 0, last_nrt = 0;
+  int col_threads = 0, row_threads = 0;
+  bool inst_dirty = true;
+  std::vector<std::pair<int, int>> where;  // batch index -> (group, slot)
+
+  ~Engine() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto &g : groups)
+      for (void *p : g.allocs) (void)hipFree(p);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+size_t esize(const Group &g) { return g.fp32 ? 8 : 16; }
+
+int dalloc(Group &g, void **p, size_t bytes) {
+  if (bytes == 0) { *p = nullptr; return OPO_OK; }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return fail(OPO_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  g.allocs.push_back(*p);
+  return OPO_OK;
+}
+
+// copy complex host data to a device buffer of the group's element type
+int upload_c(Engine &E, Group &g, void *dst, const cd *src, size_t n) {
+  if (g.fp32) {
+    std::vector<float> tmp(2 * n);
+    for (size_t i = 0; i < n; ++i) { tmp[2 * i] = (float)src[i].real(); tmp[2 * i + 1] = (float)src[i].imag(); }
+    HIPCHK(hipMemcpyAsync(dst, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+  } else {
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(cd), hipMemcpyHostToDevice, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+  }
+  return OPO_OK;
+}
+
+int download_c(Engine &E, Group &g, cd *dst, const void *src, size_t n) {
+  if (g.fp32) {
+    std::vector<float> tmp(2 * n);
+    HIPCHK(hipMemcpyAsync(tmp.data(), src, tmp.size() * sizeof(float), hipMemcpyDeviceToHost, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+    for (size_t i = 0; i < n; ++i) dst[i] = cd(tmp[2 * i], tmp[2 * i + 1]);
+  } else {
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(cd), hipMemcpyDeviceToHost, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+  }
+  return OPO_OK;
+}
+
+int validate(const opo_params *p) {
+  if (!p) return fail(OPO_ERR_INVALID, "null params");
+  if (!is_pow2(p->N) || p->N < 256 || p->N > (1 << 18))
+    return fail(OPO_ERR_INVALID, "N must be a power of two in [256, 262144]");
+  if (p->NZ < 1 || p->NRT < 1) return fail(OPO_ERR_INVALID, "NZ and NRT must be >= 1");
+  if (p->precision != OPO_FP64 && p->precision != OPO_FP32C) return fail(OPO_ERR_INVALID, "bad precision");
+  if (p->regime != OPO_CW && p->regime != OPO_NS) return fail(OPO_ERR_INVALID, "bad regime");
+  if (p->Nrts < 0) return fail(OPO_ERR_INVALID, "Nrts must be >= 0");
+  if (!(p->dz > 0) || !std::isfinite(p->dk)) return fail(OPO_ERR_INVALID, "dz must be > 0, dk finite");
+  return OPO_OK;
+}
+
+void pick_tiles(Engine &E, Group &g) {
+  const int T1 = g.N1 / 8, T2 = g.N2 / 8;
+  int ctn = E.col_threads ? E.col_threads : (E.count == 1 ? 64 : 128);
+  int rtn = E.row_threads ? E.row_threads : (E.count == 1 ? 64 : 128);
+  g.ct = std::max(1, std::min(ctn / T1, g.N2));
+  g.rt = std::max(1, std::min(rtn / T2, g.N1));
+}
+
+int make_group(Engine &E, Group &g) {
+  g.NF = E.three_eqs ? 3 : 2;
+  g.fp32 = E.fp32;
+  const int L = ilog2ll(g.N);
+  g.N1 = 1 << (L / 2);
+  g.N2 = g.N / g.N1;
+  g.lobits = (L + 1) / 2;
+  const size_t es = esize(g);
+  RCHK(dalloc(g, &g.tw1, g.N1 * es));
+  RCHK(dalloc(g, &g.tw2, g.N2 * es));
+  RCHK(dalloc(g, &g.twlo, ((size_t)1 << g.lobits) * es));
+  RCHK(dalloc(g, &g.twhi, ((size_t)g.N >> g.lobits) * es));
+  auto r1 = roots(g.N1, 1, g.N1), r2 = roots(g.N2, 1, g.N2);
+  auto rlo = roots(1LL << g.lobits, 1, g.N), rhi = roots((long long)g.N >> g.lobits, 1LL << g.lobits, g.N);
+  RCHK(upload_c(E, g, g.tw1, r1.data(), r1.size()));
+  RCHK(upload_c(E, g, g.tw2, r2.data(), r2.size()));
+  RCHK(upload_c(E, g, g.twlo, rlo.data(), rlo.size()));
+  RCHK(upload_c(E, g, g.twhi, rhi.data(), rhi.size()));
+  const int n = (int)g.members.size();
+  RCHK(dalloc(g, (void **)&g.d_insts, n * sizeof(OpoInst)));
+  RCHK(dalloc(g, (void **)&g.d_rt, n * sizeof(int)));
+  RCHK(dalloc(g, (void **)&g.d_active, n * sizeof(int)));
+  RCHK(dalloc(g, (void **)&g.d_conv, n * sizeof(int)));
+  RCHK(dalloc(g, (void **)&g.d_status, n * sizeof(int)));
+  RCHK(dalloc(g, (void **)&g.d_pref, n * sizeof(double)));
+  RCHK(dalloc(g, (void **)&g.d_partials, (size_t)n * g.N2 * sizeof(double)));
+  g.hist_len = std::max(E.NRT, 1);
+  RCHK(dalloc(g, (void **)&g.d_hist, (size_t)n * g.hist_len * sizeof(double)));
+  HIPCHK(hipMemsetAsync(g.d_hist, 0, (size_t)n * g.hist_len * sizeof(double), E.stream));
+  g.inst.resize(n);
+  for (int k = 0; k < n; ++k) {
+    Instance &I = g.inst[k];
+    RCHK(dalloc(g, &I.state, (size_t)g.NF * g.N * es));
+    RCHK(dalloc(g, &I.lin, (size_t)g.NF * g.N * es));
+    RCHK(dalloc(g, (void **)&I.nl, (size_t)E.NZ * 16 * sizeof(double)));
+  }
+  pick_tiles(E, g);
+  return OPO_OK;
+}
+
+int grow_hist(Engine &E, Group &g, int need) {
+  if (need <= g.hist_len) return OPO_OK;
+  const int n = (int)g.members.size();
+  void *p = nullptr;
+  RCHK(dalloc(g, &p, (size_t)n * need * sizeof(double)));
+  HIPCHK(hipMemsetAsync(p, 0, (size_t)n * need * sizeof(double), E.stream));
+  g.d_hist = (double *)p;
+  g.hist_len = need;
+  E.inst_dirty = true;
+  return OPO_OK;
+}
+
+// Host tables of one instance.  Every expression follows the reference line
+// it replaces, in FP64.
+int upload_instance(Engine &E, Group &g, int k, const double *w, const double *T, const double *Ap_in,
+                    size_t n_ap, const double *Ap, const double *As, const double *Ai) {
+  Instance &I = g.inst[k];
+  const opo_params &p = I.prm;
+  const int N = g.N, N1 = g.N1, NF = g.NF;
+  const size_t es = esize(g);
+  if (!w || !Ap_in) return fail(OPO_ERR_INVALID, "w and Ap_in are required");
+  const size_t need_ap = p.regime == OPO_CW ? (size_t)N : (size_t)N * p.NRT;
+  if (n_ap < need_ap) return fail(OPO_ERR_INVALID, "Ap_in too short (N for CW, N*NRT for NS)");
+  // position in the four-step frequency layout of natural frequency index q
+  auto perm = [&](int q) { return (size_t)(g.N2 * (q % N1) + q / N1); };
+
+  // linear operator, cwes3.cuh:112-125 / cwes2.cuh:108-118 with z = dz (Q8); 1/N of CUFFTscale folded in
+  std::vector<cd> lin((size_t)NF * N);
+  double D[3], b2[3], b3[3], al[3];
+  if (E.three_eqs) {
+    D[0] = (1 / p.vp - 1 / p.vi); D[1] = (1 / p.vs - 1 / p.vs); D[2] = (1 / p.vi - 1 / p.vs);
+    b2[0] = p.b2p; b2[1] = p.b2s; b2[2] = p.b2i; b3[0] = p.b3p; b3[1] = p.b3s; b3[2] = p.b3i;
+    al[0] = p.alpha_crp; al[1] = p.alpha_crs; al[2] = p.alpha_cri;
+  } else {
+    D[0] = (1 / p.vp - 1 / p.vs); D[1] = (1 / p.vs - 1 / p.vs);
+    b2[0] = p.b2p; b2[1] = p.b2s; b3[0] = p.b3p; b3[1] = p.b3s;
+    al[0] = p.alpha_crp; al[1] = p.alpha_crs;
+  }
+  const double z = p.dz;
+  for (int f = 0; f < NF; ++f) {
+    const double atten = std::exp(-0.5 * al[f] * z);
+    for (int q = 0; q < N; ++q) {
+      const double wq = w[q];
+      const double ph = z * wq * (D[f] + 0.5 * wq * b2[f] + wq * wq * b3[f] / 6);
+      lin[(size_t)f * N + perm(q)] = cd(std::cos(ph), std::sin(ph)) * (atten / N);
+    }
+  }
+  RCHK(upload_c(E, g, I.lin, lin.data(), lin.size()));
+
+  // GDD compensation, cuOPO.cu:521-553 (AddGDD functions.cuh:156-169; AddGDDTOD :173-186)
+  if (g.any_gdd) {
+    if (!I.gdd) RCHK(dalloc(g, &I.gdd, (size_t)2 * N * es));
+    std::vector<cd> gt((size_t)2 * N);
+    const double tod[2] = {p.TODscomp, p.TODicomp}, bb3[2] = {p.b3s, p.b3i};
+    for (int f = 0; f < 2; ++f)
+      for (int q = 0; q < N; ++q) {
+        const double wq = w[q];
+        double ph;
+        if (p.GDD == 0) ph = 0;
+        else if (tod[f] != 0 && bb3[f] != 0) ph = 0.5 * wq * (wq * p.GDD + wq * wq * tod[f] / 3.0);
+        else ph = 0.5 * wq * wq * p.GDD;
+        gt[(size_t)f * N + perm(q)] = cd(std::cos(ph), std::sin(ph)) * (1.0 / N);
+      }
+    RCHK(upload_c(E, g, I.gdd, gt.data(), gt.size()));
+  }
+  // EOM, functions.cuh:242-253
+  if (p.use_eom) {
+    if (!T) return fail(OPO_ERR_INVALID, "T is required when the EOM is on");
+    if (!I.eom) RCHK(dalloc(g, &I.eom, (size_t)N * es));
+    std::vector<cd> et(N);
+    for (int j = 0; j < N; ++j) {
+      const double a = p.mod_depth * std::sin(2 * kPi * p.fpm * T[j]);
+      et[j] = cd(std::cos(a), std::sin(a));
+    }
+    RCHK(upload_c(E, g, I.eom, et.data(), et.size()));
+  }
+  // nonlinear coefficients per slice, z accumulated as cwes3.cuh:149,219
+  std::vector<double> nl((size_t)E.NZ * 16, 0.0);
+  {
+    double zz = 0;
+    const double h = p.dz / 2;
+    for (int s = 0; s < E.NZ; ++s) {
+      double *q = nl.data() + 16 * s;
+      const double za = zz, zb = zz + p.dz / 4, zc = zz + p.dz / 2;
+      auto ex = [](double a) { return cd(std::cos(a), std::sin(a)); };
+      if (E.three_eqs) {
+        // dAdz (cwes3.cuh:37-39): pump carries e^{-i dk z}, signal/idler e^{+i dk z}
+        const cd em = (ex(-p.dk * za) + 2.0 * ex(-p.dk * zb)) + 2.0 * ex(-p.dk * zb) + ex(-p.dk * zc);
+        const cd ep = (ex(+p.dk * za) + 2.0 * ex(+p.dk * zb)) + 2.0 * ex(+p.dk * zb) + ex(+p.dk * zc);
+        const cd C[3] = {cd(0, p.kp) * em * h / 6.0, cd(0, p.ks) * ep * h / 6.0, cd(0, p.ki) * ep * h / 6.0};
+        for (int j = 0; j < 3; ++j) { q[2 * j] = C[j].real(); q[2 * j + 1] = C[j].imag(); }
+      } else {
+        // dAdz (cwes2.cuh:37-38): pump e^{+i dk z}, signal e^{-i dk z}
+        const double zs[3] = {za, zb, zc};
+        for (int j = 0; j < 3; ++j) {
+          const cd a = ex(+p.dk * zs[j]), b = ex(-p.dk * zs[j]);
+          q[2 * j] = a.real(); q[2 * j + 1] = a.imag();
+          q[6 + 2 * j] = b.real(); q[6 + 2 * j + 1] = b.imag();
+        }
+      }
+      zz += p.dz;
+    }
+  }
+  HIPCHK(hipMemcpyAsync(I.nl, nl.data(), nl.size() * sizeof(double), hipMemcpyHostToDevice, E.stream));
+  HIPCHK(hipStreamSynchronize(E.stream));
+  // pump (Ap_in): N (CW) or N*NRT (NS) complex
+  if (!I.pump || I.pump_len != (long long)need_ap) {
+    RCHK(dalloc(g, &I.pump, need_ap * es));
+    I.pump_len = (long long)need_ap;
+  }
+  RCHK(upload_c(E, g, I.pump, reinterpret_cast<const cd *>(Ap_in), need_ap));
+  // fields
+  std::vector<cd> st((size_t)NF * N, cd(0, 0));
+  for (int j = 0; j < N; ++j) {
+    if (Ap) st[j] = cd(Ap[2 * j], Ap[2 * j + 1]);
+    if (As) st[(size_t)N + j] = cd(As[2 * j], As[2 * j + 1]);
+    if (NF == 3 && Ai) st[(size_t)2 * N + j] = cd(Ai[2 * j], Ai[2 * j + 1]);
+  }
+  RCHK(upload_c(E, g, I.state, st.data(), st.size()));
+  I.uploaded = true;
+  E.inst_dirty = true;
+  return OPO_OK;
+}
+
+int sync_insts(Engine &E) {
+  if (!E.inst_dirty) return OPO_OK;
+  for (auto &g : E.groups) {
+    const int n = (int)g.members.size();
+    std::vector<OpoInst> h(n);
+    for (int k = 0; k < n; ++k) {
+      Instance &I = g.inst[k];
+      const opo_params &p = I.prm;
+      OpoInst &d = h[k];
+      std::memset(&d, 0, sizeof(d));
+      d.state = I.state; d.lin = I.lin; d.gdd = I.gdd; d.eom = p.use_eom ? I.eom : nullptr; d.nl = I.nl;
+      d.pump = I.pump; d.pump_len = I.pump_len;
+      for (int f = 0; f < 3; ++f) d.save[f] = E.save_enabled ? I.save[f] : nullptr;
+      d.save_first = p.regime == OPO_CW ? p.NRT - p.Nrts : 0;
+      d.save_slots = p.regime == OPO_CW ? p.Nrts : p.NRT;
+      d.nrt = p.NRT;
+      d.regime = p.regime;
+      d.h = p.dz / 2;
+      d.kp = p.kp; d.ks = p.ks; d.ki = p.ki;
+      d.sqrt_rs = std::sqrt(p.Rs); d.sqrt_ri = std::sqrt(p.Ri);  // AddPhase, functions.cuh:145
+      d.mir[0] = std::cos(kPi * p.delta); d.mir[1] = std::sin(kPi * p.delta);
+      d.rt = g.d_rt + k; d.active = g.d_active + k; d.partials = g.d_partials + (size_t)k * g.N2;
+      d.power_hist = g.d_hist + (size_t)k * g.hist_len; d.hist_len = g.hist_len;
+      d.conv_rt = g.d_conv + k; d.p_ref = g.d_pref + k; d.status = g.d_status + k;
+      d.snap = I.snap;
+    }
+    HIPCHK(hipMemcpyAsync(g.d_insts, h.data(), n * sizeof(OpoInst), hipMemcpyHostToDevice, E.stream));
+  }
+  HIPCHK(hipStreamSynchronize(E.stream));
+  E.inst_dirty = false;
+  return OPO_OK;
+}
+
+// ------------------------------------------------------------ stage emitters
+
+int emit_col(Engine &E, Group &g, unsigned ops, int load, int inv, int fwd, int store, int s_end, int s_start,
+             int pump_off, int snap_every) {
+  opo::ColArgs a{};
+  a.tw1 = g.tw1; a.twlo = g.twlo; a.twhi = g.twhi; a.lobits = g.lobits;
+  a.N2 = g.N2; a.log2ct = ilog2ll(g.ct);
+  a.ops = ops; a.load_mask = load; a.inv_mask = inv; a.fwd_mask = fwd; a.store_mask = store;
+  a.s_end = s_end; a.s_start = s_start; a.pump_off = pump_off; a.three_eqs = E.three_eqs;
+  a.snap_every = snap_every;
+  const int threads_ct = g.ct;
+  int e = opo::launch_col(g.fp32, g.N1, g.NF, threads_ct, (int)g.members.size(), g.d_insts, a, E.stream);
+  if (e != 0) return fail(OPO_ERR_HIP, std::string("col_stage launch: ") + hipGetErrorString((hipError_t)e));
+  return OPO_OK;
+}
+
+int emit_row(Engine &E, Group &g, bool gdd, int mask) {
+  opo::RowArgs a{};
+  a.tw2 = g.tw2; a.N1 = g.N1; a.log2rt = ilog2ll(g.rt); a.use_gdd = gdd ? 1 : 0;
+  int fl[3] = {0, 0, 0}, nf = 0;
+  for (int f = 0; f < g.NF; ++f)
+    if ((mask >> f) & 1) fl[nf++] = f;
+  a.nf = nf; a.f0 = fl[0]; a.f1 = fl[1]; a.f2 = fl[2];
+  int e = opo::launch_row(g.fp32, g.N2, g.NF, g.rt, (int)g.members.size(), 0, g.d_insts, a, E.stream);
+  if (e != 0) return fail(OPO_ERR_HIP, std::string("row_stage launch: ") + hipGetErrorString((hipError_t)e));
+  return OPO_OK;
+}
+
+int emit_tick(Engine &E, Group &g, int check_every, double tol) {
+  opo::TickArgs a{};
+  a.count = (int)g.members.size(); a.tiles = g.N2 / g.ct; a.check_every = check_every; a.tol = tol;
+  int e = opo::launch_tick(g.d_insts, a, E.stream);
+  if (e != 0) return fail(OPO_ERR_HIP, std::string("tick launch: ") + hipGetErrorString((hipError_t)e));
+  return OPO_OK;
+}
+
+int all_mask(const Group &g) { return (1 << g.NF) - 1; }
+int si_mask(const Group &g) { return g.NF == 3 ? 6 : 2; }
+
+// start of a run: time-domain fields -> pump re-injection, NL(z_0), forward columns
+int emit_start(Engine &E, Group &g) {
+  const int A = all_mask(g);
+  return emit_col(E, g, opo::OP_PUMP | opo::OP_NL_START, A, 0, A, A, 0, 0, 0, 0);
+}
+
+// one round trip after emit_start (or after a previous non-final round trip)
+int emit_rt_body(Engine &E, Group &g, bool last, int check_every, double tol) {
+  using namespace opo;
+  const int A = all_mask(g), SI = si_mask(g), NZ = E.NZ;
+  for (int s = 0; s < NZ; ++s) {
+    RCHK(emit_row(E, g, false, A));
+    if (s < NZ - 1) RCHK(emit_col(E, g, OP_NL_END | OP_NL_START, A, A, A, A, s, s + 1, 0, 0));
+  }
+  const unsigned tail = OP_MIRROR | OP_SAVE | OP_POWER | OP_SNAP;
+  const unsigned cont = last ? 0u : (OP_PUMP | OP_NL_START);
+  const int fwd = last ? 0 : A;
+  if (!g.any_gdd) {
+    RCHK(emit_col(E, g, OP_NL_END | OP_EOM | tail | cont, A, A, fwd, A, NZ - 1, 0, 1, check_every));
+  } else {
+    RCHK(emit_col(E, g, OP_NL_END | OP_EOM, A, A, SI, A, NZ - 1, 0, 0, 0));
+    RCHK(emit_row(E, g, true, SI));
+    RCHK(emit_col(E, g, tail | cont, A, SI, fwd, A, 0, 0, 1, check_every));
+  }
+  return emit_tick(E, g, check_every, tol);
+}
+
+int reset_counters(Engine &E, int rt0) {
+  for (auto &g : E.groups) {
+    const int n = (int)g.members.size();
+    std::vector<int> rt(n, rt0), act(n, 1), conv(n, -1), stat(n, 0);
+    std::vector<double> pref(n, 0.0);
+    HIPCHK(hipMemcpyAsync(g.d_rt, rt.data(), n * sizeof(int), hipMemcpyHostToDevice, E.stream));
+    HIPCHK(hipMemcpyAsync(g.d_active, act.data(), n * sizeof(int), hipMemcpyHostToDevice, E.stream));
+    HIPCHK(hipMemcpyAsync(g.d_conv, conv.data(), n * sizeof(int), hipMemcpyHostToDevice, E.stream));
+    HIPCHK(hipMemcpyAsync(g.d_status, stat.data(), n * sizeof(int), hipMemcpyHostToDevice, E.stream));
+    HIPCHK(hipMemcpyAsync(g.d_pref, pref.data(), n * sizeof(double), hipMemcpyHostToDevice, E.stream));
+  }
+  HIPCHK(hipStreamSynchronize(E.stream));
+  return OPO_OK;
+}
+
+int build_graph(Engine &E, int check_every, double tol) {
+  const int key = check_every * 1000003 + (int)(tol * 1e12) + (E.save_enabled ? 7 : 0);
+  if (E.gexec && E.graph_key == key) return OPO_OK;
+  if (E.gexec) { (void)hipGraphExecDestroy(E.gexec); E.gexec = nullptr; }
+  if (E.graph) { (void)hipGraphDestroy(E.graph); E.graph = nullptr; }
+  HIPCHK(hipStreamBeginCapture(E.stream, hipStreamCaptureModeThreadLocal));
+  int rc = OPO_OK;
+  for (auto &g : E.groups) {
+    rc = emit_rt_body(E, g, false, check_every, tol);
+    if (rc != OPO_OK) break;
+  }
+  hipGraph_t gr = nullptr;
+  hipError_t ee = hipStreamEndCapture(E.stream, &gr);
+  if (rc != OPO_OK) { if (gr) (void)hipGraphDestroy(gr); return rc; }
+  if (ee != hipSuccess) return fail(OPO_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
+  E.graph = gr;
+  HIPCHK(hipGraphInstantiate(&E.gexec, E.graph, nullptr, nullptr, 0));
+  E.graph_key = key;
+  return OPO_OK;
+}
+
+int any_active(Engine &E, bool *out) {
+  *out = false;
+  for (auto &g : E.groups) {
+    const int n = (int)g.members.size();
+    std::vector<int> act(n);
+    HIPCHK(hipMemcpyAsync(act.data(), g.d_active, n * sizeof(int), hipMemcpyDeviceToHost, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+    for (int v : act) if (v) { *out = true; return OPO_OK; }
+  }
+  return OPO_OK;
+}
+
+// n_rt round trips starting at rt0 (every instance), ending in the time domain.
+int engine_run(Engine &E, int rt0, int n_rt, int check_every, double tol) {
+  if (n_rt <= 0) return OPO_OK;
+  for (auto &g : E.groups) {
+    for (auto &I : g.inst)
+      if (!I.uploaded) return fail(OPO_ERR_STATE, "run before upload");
+    RCHK(grow_hist(E, g, rt0 + n_rt));
+    if (check_every > 0)
+      for (auto &I : g.inst)
+        if (!I.snap) { RCHK(dalloc(g, &I.snap, (size_t)g.NF * g.N * esize(g))); E.inst_dirty = true; }
+  }
+  RCHK(sync_insts(E));
+  RCHK(reset_counters(E, rt0));
+  const bool graph = E.use_graph && n_rt > 2;
+  if (graph) RCHK(build_graph(E, check_every, tol));
+  HIPCHK(hipEventRecord(E.ev0, E.stream));
+  for (auto &g : E.groups) RCHK(emit_start(E, g));
+  int done = 0;
+  bool alive = true;
+  while (done < n_rt - 1 && alive) {
+    const int seg = check_every > 0 ? std::min(check_every, n_rt - 1 - done) : n_rt - 1 - done;
+    for (int r = 0; r < seg; ++r) {
+      if (graph) HIPCHK(hipGraphLaunch(E.gexec, E.stream));
+      else for (auto &g : E.groups) RCHK(emit_rt_body(E, g, false, check_every, tol));
+    }
+    done += seg;
+    if (check_every > 0) RCHK(any_active(E, &alive));
+  }
+  if (alive)
+    for (auto &g : E.groups) RCHK(emit_rt_body(E, g, true, check_every, tol));
+  HIPCHK(hipEventRecord(E.ev1, E.stream));
+  HIPCHK(hipEventSynchronize(E.ev1));
+  HIPCHK(hipEventElapsedTime(&E.last_ms, E.ev0, E.ev1));
+  E.last_rt0 = rt0; E.last_nrt = n_rt;
+  // surface a launch-time fault immediately
+  HIPCHK(hipGetLastError());
+  return OPO_OK;
+}
+
+int engine_init(Engine &E, const opo_params *params, int count) {
+  if (count < 1) return fail(OPO_ERR_INVALID, "count must be >= 1");
+  for (int k = 0; k < count; ++k) RCHK(validate(&params[k]));
+  const opo_params &p0 = params[0];
+  for (int k = 1; k < count; ++k) {
+    const opo_params &p = params[k];
+    if (p.three_eqs != p0.three_eqs || p.regime != p0.regime || p.NZ != p0.NZ || p.precision != p0.precision ||
+        p.device != p0.device)
+      return fail(OPO_ERR_INVALID, "batch instances must share three_eqs, regime, NZ, precision and device");
+  }
+  E.count = count; E.NZ = p0.NZ; E.three_eqs = p0.three_eqs ? 1 : 0; E.regime = p0.regime;
+  E.fp32 = p0.precision == OPO_FP32C; E.device = p0.device;
+  E.NRT = 1;
+  for (int k = 0; k < count; ++k) E.NRT = std::max(E.NRT, params[k].NRT);
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (E.device < 0 || E.device >= ndev) return fail(OPO_ERR_INVALID, "device ordinal out of range");
+  HIPCHK(hipSetDevice(E.device));
+  HIPCHK(hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreate(&E.ev0));
+  HIPCHK(hipEventCreate(&E.ev1));
+  // group by N, preserving order
+  E.where.assign(count, {-1, -1});
+  for (int k = 0; k < count; ++k) {
+    int gi = -1;
+    for (size_t j = 0; j < E.groups.size(); ++j)
+      if (E.groups[j].N == params[k].N) gi = (int)j;
+    if (gi < 0) { E.groups.emplace_back(); gi = (int)E.groups.size() - 1; E.groups[gi].N = params[k].N; }
+    E.where[k] = {gi, (int)E.groups[gi].members.size()};
+    E.groups[gi].members.push_back(k);
+  }
+  for (auto &g : E.groups) {
+    for (int k : g.members) if (params[k].GDD != 0) g.any_gdd = true;
+    RCHK(make_group(E, g));
+    for (size_t j = 0; j < g.members.size(); ++j) g.inst[j].prm = params[g.members[j]];
+  }
+  return OPO_OK;
+}
+
+}  // namespace
+
+// ================================================================== C ABI
+
+struct opo_solver {
+  Engine E;
+};
+struct opo_batch {
+  Engine E;
+};
+
+extern "C" {
+
+const char *opo_strerror(int status) {
+  switch (status) {
+    case OPO_OK: return "success";
+    case OPO_ERR_INVALID: return "invalid argument";
+    case OPO_ERR_HIP: return "HIP runtime error";
+    case OPO_ERR_NOMEM: return "device out of memory";
+    case OPO_ERR_STATE: return "call out of order";
+    case OPO_ERR_DIVERGED: return "non-finite intracavity power";
+  }
+  return "unknown status";
+}
+
+const char *opo_last_error(void) { return g_err.c_str(); }
+
+int opo_device_count(int *n) {
+  if (!n) return fail(OPO_ERR_INVALID, "null");
+  HIPCHK(hipGetDeviceCount(n));
+  return OPO_OK;
+}
+
+int opo_create(const opo_params *p, opo_solver **out) {
+  if (!out) return fail(OPO_ERR_INVALID, "null out");
+  *out = nullptr;
+  std::unique_ptr<opo_solver> s(new opo_solver());
+  RCHK(engine_init(s->E, p, 1));
+  *out = s.release();
+  return OPO_OK;
+}
+
+int opo_destroy(opo_solver *s) {
+  delete s;
+  return OPO_OK;
+}
+
+int opo_upload(opo_solver *s, const double *w, const double *T, const double *Ap_in, size_t n_ap, const double *Ap,
+               const double *As, const double *Ai) {
+  if (!s) return fail(OPO_ERR_INVALID, "null solver");
+  Engine &E = s->E;
+  HIPCHK(hipSetDevice(E.device));
+  return upload_instance(E, E.groups[0], 0, w, T, Ap_in, n_ap, Ap, As, Ai);
+}
+
+int opo_set_fields(opo_solver *s, const double *Ap, const double *As, const double *Ai) {
+  if (!s) return fail(OPO_ERR_INVALID, "null solver");
+  Engine &E = s->E;
+  Group &g = E.groups[0];
+  if (!g.inst[0].uploaded) return fail(OPO_ERR_STATE, "set_fields before upload");
+  std::vector<cd> st((size_t)g.NF * g.N, cd(0, 0));
+  for (int j = 0; j < g.N; ++j) {
+    if (Ap) st[j] = cd(Ap[2 * j], Ap[2 * j + 1]);
+    if (As) st[(size_t)g.N + j] = cd(As[2 * j], As[2 * j + 1]);
+    if (g.NF == 3 && Ai) st[(size_t)2 * g.N + j] = cd(Ai[2 * j], Ai[2 * j + 1]);
+  }
+  return upload_c(E, g, g.inst[0].state, st.data(), st.size());
+}
+
+int opo_enable_save(opo_solver *s, int enable) {
+  if (!s) return fail(OPO_ERR_INVALID, "null solver");
+  Engine &E = s->E;
+  Group &g = E.groups[0];
+  Instance &I = g.inst[0];
+  if (enable) {
+    const size_t slots = I.prm.regime == OPO_CW ? (size_t)I.prm.Nrts : (size_t)I.prm.NRT;
+    for (int f = 0; f < g.NF; ++f)
+      if (!I.save[f]) {
+        RCHK(dalloc(g, &I.save[f], slots * g.N * esize(g)));
+        HIPCHK(hipMemsetAsync(I.save[f], 0, slots * g.N * esize(g), E.stream));
+      }
+  }
+  E.save_enabled = enable != 0;
+  E.inst_dirty = true;
+  E.graph_key = -1;
+  return OPO_OK;
+}
+
+int opo_set_tiles(opo_solver *s, int col_threads, int row_threads) {
+  if (!s) return fail(OPO_ERR_INVALID, "null solver");
+  for (int v : {col_threads, row_threads})
+    if (v != 0 && v != 32 && v != 64 && v != 128 && v != 256) return fail(OPO_ERR_INVALID, "tile threads in {64,128,256}");
+  s->E.col_threads = col_threads; s->E.row_threads = row_threads;
+  for (auto &g : s->E.groups) pick_tiles(s->E, g);
+  s->E.graph_key = -1;
+  return OPO_OK;
+}
+
+int opo_use_graph(opo_solver *s, int enable) {
+  if (!s) return fail(OPO_ERR_INVALID, "null solver");
+  s->E.use_graph = enable != 0;
+  return OPO_OK;
+}
+
+int opo_crystal_pass(opo_solver *s) {
+  using namespace opo;
+  if (!s) return fail(OPO_ERR_INVALID, "null solver");
+  Engine &E = s->E;
+  Group &g = E.groups[0];
+  if (!g.inst[0].uploaded) return fail(OPO_ERR_STATE, "crystal_pass before upload");
+  RCHK(sync_insts(E));
+  RCHK(reset_counters(E, 0));
+  const int A = all_mask(g);
+  RCHK(emit_col(E, g, OP_NL_START, A, 0, A, A, 0, 0, 0, 0));
+  for (int sl = 0; sl < E.NZ; ++sl) {
+    RCHK(emit_row(E, g, false, A));
+    if (sl < E.NZ - 1) RCHK(emit_col(E, g, OP_NL_END | OP_NL_START, A, A, A, A, sl, sl + 1, 0, 0));
+  }
+  RCHK(emit_col(E, g, OP_NL_END, A, A, 0, A, E.NZ - 1, 0, 0, 0));
+  HIPCHK(hipStreamSynchronize(E.stream));
+  return OPO_OK;
+}
+
+int opo_linear_step(opo_solver *s) {
+  if (!s) return fail(OPO_ERR_INVALID, "null solver");
+  Engine &E = s->E;
+  Group &g = E.groups[0];
+  if (!g.inst[0].uploaded) return fail(OPO_ERR_STATE, "linear_step before upload");
+  RCHK(sync_insts(E));
+  RCHK(reset_counters(E, 0));
+  const int A = all_mask(g);
+  RCHK(emit_col(E, g, 0, A, 0, A, A, 0, 0, 0, 0));
+  RCHK(emit_row(E, g, false, A));
+  RCHK(emit_col(E, g, 0, A, A, 0, A, 0, 0, 0, 0));
+  HIPCHK(hipStreamSynchronize(E.stream));
+  return OPO_OK;
+}
+
+int opo_nl_half(opo_solver *s, int slice, int n_times) {
+  if (!s) return fail(OPO_ERR_INVALID, "null solver");
+  Engine &E = s->E;
+  Group &g = E.groups[0];
+  if (!g.inst[0].uploaded) return fail(OPO_ERR_STATE, "nl_half before upload");
+  if (slice < 0 || slice >= E.NZ) return fail(OPO_ERR_INVALID, "slice out of range");
+  RCHK(sync_insts(E));
+  RCHK(reset_counters(E, 0));
+  const int A = all_mask(g);
+  for (int r = 0; r < n_times; ++r) RCHK(emit_col(E, g, opo::OP_NL_START, A, 0, 0, A, 0, slice, 0, 0));
+  HIPCHK(hipStreamSynchronize(E.stream));
+  return OPO_OK;
+}
+
+int opo_boundary(opo_solver *s) {
+  using namespace opo;
+  if (!s) return fail(OPO_ERR_INVALID, "null solver");
+  Engine &E = s->E;
+  Group &g = E.groups[0];
+  if (!g.inst[0].uploaded) return fail(OPO_ERR_STATE, "boundary before upload");
+  RCHK(sync_insts(E));
+  RCHK(reset_counters(E, 0));
+  const int A = all_mask(g), SI = si_mask(g);
+  if (g.any_gdd) {
+    RCHK(emit_col(E, g, OP_EOM, A, 0, SI, A, 0, 0, 0, 0));
+    RCHK(emit_row(E, g, true, SI));
+    RCHK(emit_col(E, g, OP_MIRROR, A, SI, 0, A, 0, 0, 0, 0));
+  } else {
+    RCHK(emit_col(E, g, OP_EOM | OP_MIRROR, A, 0, 0, A, 0, 0, 0, 0));
+  }
+  HIPCHK(hipStreamSynchronize(E.stream));
+  return OPO_OK;
+}
+
+int opo_run(opo_solver *s, int rt0, int n_rt) {
+  if (!s) return fail(OPO_ERR_INVALID, "null solver");
+  if (rt0 < 0 || n_rt < 0) return fail(OPO_ERR_INVALID, "negative round-trip index");
+  HIPCHK(hipSetDevice(s->E.device));
+  return engine_run(s->E, rt0, n_rt, 0, 0.0);
+}
+
+int opo_download(opo_solver *s, double *Ap, double *As, double *Ai) {
+  if (!s) return fail(OPO_ERR_INVALID, "null solver");
+  Engine &E = s->E;
+  Group &g = E.groups[0];
+  std::vector<cd> st((size_t)g.NF * g.N);
+  RCHK(download_c(E, g, st.data(), g.inst[0].state, st.size()));
+  if (Ap) std::memcpy(Ap, st.data(), g.N * sizeof(cd));
+  if (As) std::memcpy(As, st.data() + g.N, g.N * sizeof(cd));
+  if (Ai && g.NF == 3) std::memcpy(Ai, st.data() + 2 * (size_t)g.N, g.N * sizeof(cd));
+  return OPO_OK;
+}
+
+int opo_download_saved(opo_solver *s, double *sp, double *ss, double *si) {
+  if (!s) return fail(OPO_ERR_INVALID, "null solver");
+  Engine &E = s->E;
+  Group &g = E.groups[0];
+  Instance &I = g.inst[0];
+  const size_t slots = I.prm.regime == OPO_CW ? (size_t)I.prm.Nrts : (size_t)I.prm.NRT;
+  double *dst[3] = {sp, ss, si};
+  for (int f = 0; f < g.NF; ++f) {
+    if (!dst[f]) continue;
+    if (!I.save[f]) return fail(OPO_ERR_STATE, "saving was not enabled");
+    RCHK(download_c(E, g, reinterpret_cast<cd *>(dst[f]), I.save[f], slots * g.N));
+  }
+  return OPO_OK;
+}
+
+int opo_power_history(opo_solver *s, double *out, int n) {
+  if (!s || !out) return fail(OPO_ERR_INVALID, "null");
+  Engine &E = s->E;
+  Group &g = E.groups[0];
+  if (n > E.last_nrt) n = E.last_nrt;
+  HIPCHK(hipMemcpy(out, g.d_hist + E.last_rt0, n * sizeof(double), hipMemcpyDeviceToHost));
+  return OPO_OK;
+}
+
+int opo_last_run_ms(opo_solver *s, double *ms) {
+  if (!s || !ms) return fail(OPO_ERR_INVALID, "null");
+  *ms = s->E.last_ms;
+  return OPO_OK;
+}
+
+// ---------------------------------------------------------------- batches
+
+int opo_batch_create(const opo_params *params, int count, opo_batch **out) {
+  if (!out || !params) return fail(OPO_ERR_INVALID, "null");
+  *out = nullptr;
+  std::unique_ptr<opo_batch> b(new opo_batch());
+  RCHK(engine_init(b->E, params, count));
+  *out = b.release();
+  return OPO_OK;
+}
+
+int opo_batch_destroy(opo_batch *b) {
+  delete b;
+  return OPO_OK;
+}
+
+int opo_batch_upload(opo_batch *b, int idx, const double *w, const double *T, const double *Ap_in, size_t n_ap,
+                     const double *Ap, const double *As, const double *Ai) {
+  if (!b || idx < 0 || idx >= b->E.count) return fail(OPO_ERR_INVALID, "bad batch index");
+  Engine &E = b->E;
+  HIPCHK(hipSetDevice(E.device));
+  auto [gi, k] = E.where[idx];
+  return upload_instance(E, E.groups[gi], k, w, T, Ap_in, n_ap, Ap, As, Ai);
+}
+
+int opo_batch_set_tiles(opo_batch *b, int col_threads, int row_threads) {
+  if (!b) return fail(OPO_ERR_INVALID, "null batch");
+  for (int v : {col_threads, row_threads})
+    if (v != 0 && v != 32 && v != 64 && v != 128 && v != 256) return fail(OPO_ERR_INVALID, "tile threads in {64,128,256}");
+  b->E.col_threads = col_threads; b->E.row_threads = row_threads;
+  for (auto &g : b->E.groups) pick_tiles(b->E, g);
+  b->E.graph_key = -1;
+  return OPO_OK;
+}
+
+int opo_batch_run(opo_batch *b, int max_rt, int check_every, double tol, int *rts_out) {
+  if (!b) return fail(OPO_ERR_INVALID, "null batch");
+  if (max_rt < 1) return fail(OPO_ERR_INVALID, "max_rt must be >= 1");
+  Engine &E = b->E;
+  HIPCHK(hipSetDevice(E.device));
+  if (tol <= 0) check_every = 0;
+  RCHK(engine_run(E, 0, max_rt, check_every, tol));
+  int rc = OPO_OK;
+  for (auto &g : E.groups) {
+    const int n = (int)g.members.size();
+    std::vector<int> conv(n), stat(n);
+    HIPCHK(hipMemcpy(conv.data(), g.d_conv, n * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(stat.data(), g.d_status, n * sizeof(int), hipMemcpyDeviceToHost));
+    for (int k = 0; k < n; ++k) {
+      if (rts_out) rts_out[g.members[k]] = stat[k] != 0 ? -2 : conv[k];
+      if (stat[k] != 0) rc = OPO_ERR_DIVERGED;
+    }
+  }
+  if (rc != OPO_OK) return fail(rc, "non-finite intracavity power in at least one instance");
+  return OPO_OK;
+}
+
+int opo_batch_download(opo_batch *b, int idx, double *Ap, double *As, double *Ai) {
+  if (!b || idx < 0 || idx >= b->E.count) return fail(OPO_ERR_INVALID, "bad batch index");
+  Engine &E = b->E;
+  auto [gi, k] = E.where[idx];
+  Group &g = E.groups[gi];
+  int conv = -1;
+  HIPCHK(hipMemcpy(&conv, g.d_conv + k, sizeof(int), hipMemcpyDeviceToHost));
+  const void *src = (conv >= 0 && g.inst[k].snap) ? g.inst[k].snap : g.inst[k].state;
+  std::vector<cd> st((size_t)g.NF * g.N);
+  RCHK(download_c(E, g, st.data(), src, st.size()));
+  if (Ap) std::memcpy(Ap, st.data(), g.N * sizeof(cd));
+  if (As) std::memcpy(As, st.data() + g.N, g.N * sizeof(cd));
+  if (Ai && g.NF == 3) std::memcpy(Ai, st.data() + 2 * (size_t)g.N, g.N * sizeof(cd));
+  return OPO_OK;
+}
+
+int opo_batch_power(opo_batch *b, double *out) {
+  if (!b || !out) return fail(OPO_ERR_INVALID, "null");
+  Engine &E = b->E;
+  for (auto &g : E.groups) {
+    const int n = (int)g.members.size();
+    std::vector<int> rt(n);
+    HIPCHK(hipMemcpy(rt.data(), g.d_rt, n * sizeof(int), hipMemcpyDeviceToHost));
+    for (int k = 0; k < n; ++k) {
+      const int last = std::max(0, std::min(rt[k], g.hist_len) - 1);
+      HIPCHK(hipMemcpy(out + g.members[k], g.d_hist + (size_t)k * g.hist_len + last, sizeof(double),
+                       hipMemcpyDeviceToHost));
+    }
+  }
+  return OPO_OK;
+}
+
+int opo_batch_last_run_ms(opo_batch *b, double *ms) {
+  if (!b || !ms) return fail(OPO_ERR_INVALID, "null");
+  *ms = b->E.last_ms;
+  return OPO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,181 @@
+/*
+ * opo.h -- C ABI of libcuopo_amd.so, the MI355X (gfx950) split-step Fourier
+ * round-trip solver for the three-wave-mixing coupled-wave equations.
+ *
+ * Plain C types only: sizes are int/size_t, fields are interleaved complex128
+ * (double re, double im -- the layout of std::complex<double>), host buffers
+ * are caller-owned and copied, device buffers are owned by the handle.  The
+ * library never exits the process: every entry point returns an opo_status
+ * (the cuOPO executable maps a non-zero status to the reference's
+ * "Error: file:line" print and exit(1), src/headers/common.cuh:6-16).
+ *
+ * Reference interfaces replaced (paths relative to the cuOPO source tree):
+ *   opo_derive ........... setup block of main(), src/cuOPO.cu:92-269
+ *   opo_grids/_long ...... linspace+fftshift grids, src/cuOPO.cu:170-189
+ *   opo_pump_input ....... input_field_T, src/headers/functions.cuh:77-100
+ *   opo_noise ............ NoiseGeneratorCPU, src/headers/functions.cuh:58-71
+ *   opo_create/upload .... device allocation + H2D copies, src/cuOPO.cu:379-442
+ *   opo_crystal_pass ..... EvolutionInCrystal, src/headers/cwes3.cuh:139-225
+ *                          and src/headers/cwes2.cuh:132-215
+ *   opo_linear_step ...... IFFT/CUFFTscale/LinearOperator/FFT, cwes3.cuh:175-194
+ *   opo_nl_half .......... dAdz/LinealCombination/rk4 chain, cwes3.cuh:153-172
+ *   opo_boundary ......... EOM + GDD + AddPhase, src/cuOPO.cu:509-566
+ *   opo_run .............. round-trip loop, src/cuOPO.cu:474-610
+ *   opo_download* ........ SaveVectorComplexGPU D2H, src/headers/files.cuh:65-75
+ *   opo_destroy .......... teardown, src/cuOPO.cu:645-668
+ *   opo_batch_* .......... the serial sweep of src/cuOPO.sh:91-175, batched
+ * Entry points marked "no reference counterpart" are additions.
+ */
+#ifndef OPO_H
+#define OPO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  OPO_OK = 0,
+  OPO_ERR_INVALID = -1,  /* bad argument or unsupported size */
+  OPO_ERR_HIP = -2,      /* HIP runtime error (message via opo_last_error) */
+  OPO_ERR_NOMEM = -3,    /* device allocation failed */
+  OPO_ERR_STATE = -4,    /* call out of order (e.g. run before upload) */
+  OPO_ERR_DIVERGED = -5  /* non-finite intracavity power detected */
+} opo_status;
+
+enum { OPO_PPLN = 0, OPO_SPPLT = 1 };        /* -DPPLN / -DSPPLT   (src/cuOPO.cu:36-41) */
+enum { OPO_CW = 0, OPO_NS = 1 };             /* -DCW_OPO / -DNS_OPO (src/cuOPO.cu:65-70) */
+enum { OPO_FP64 = 0, OPO_FP32C = 1 };        /* FP32-complex variant: no reference counterpart */
+enum { OPO_PM_REFERENCE = 0, OPO_PM_THREE_WAVE = 1 };
+
+/* argv[1..12] of the reference executable plus its compile-time switches
+ * (src/cuOPO.cu:36-47,92-267) and the opt-in extensions (ls, pm_mode). */
+typedef struct {
+  int three_eqs, regime, crystal;
+  int N, NZ, NRT;
+  double lp_nm, temp_arg, Lambda, Nth, R_pct, delta, gdd_pct, tod_pct;
+  int upm;
+  double md, fm, lcr_mm;
+  double ls_um;  /* 0 => ls = 2*lp as in the reference (src/cuOPO.cu:93) */
+  int pm_mode;   /* OPO_PM_REFERENCE => functions.cuh:23-54 scan */
+} opo_args;
+
+/* Derived physical scalars (src/cuOPO.cu:92-269). */
+typedef struct {
+  int three_eqs, regime, crystal, N, NZ, NRT;
+  double lp, ls, li, Lambda, Temp, Lcr;
+  double np, ns, ni, vp, vs, vi, b2p, b2s, b2i, b3p, b3s, b3i, kp, ks, ki;
+  double dk, dkp, Lcav, Rs, Ri, alphas, alphai, t_rt, FSR, finesse, lw;
+  double delta, epsilon, GDD, TODscomp, TODicomp, dz, dT, dF;
+  double deff, dQ, alpha_crp, alpha_crs, alpha_cri;
+  double waist, spot, Ith, Pth, Nth, Inten, Power, Ap0, GammaL;
+  int use_eom;
+  double mod_depth, df, fpm;
+  int Nrts;
+  size_t SIZEL;
+  double T_width, dFp, FWHM, sigmap;
+} opo_derived;
+
+/* What the solver needs: the scalar arguments of EvolutionInCrystal
+ * (cwes3.cuh:139) plus the boundary constants of the round-trip loop. */
+typedef struct {
+  int three_eqs, regime, N, NZ, NRT;
+  int precision;              /* OPO_FP64 or OPO_FP32C */
+  double dz, dk, kp, ks, ki;
+  double vp, vs, vi, b2p, b2s, b2i, b3p, b3s, b3i;
+  double alpha_crp, alpha_crs, alpha_cri;
+  double Rs, Ri, delta;
+  double GDD, TODscomp, TODicomp;
+  int use_eom;
+  double mod_depth, fpm;
+  int Nrts;                   /* CW: number of trailing round trips saved (64) */
+  int device;                 /* HIP device ordinal */
+} opo_params;
+
+/* ---------------- host-side setup (product code, FP64) ---------------- */
+/* Returns OPO_ERR_INVALID when the EOM is on and Nth < 1 (the reference takes
+ * sqrtf(Nth-1), src/cuOPO.cu:267). */
+int opo_derive(const opo_args *a, opo_derived *d);
+void opo_params_from_derived(const opo_derived *d, int precision, opo_params *p);
+void opo_grids(const opo_derived *d, double *T /*N*/, double *w /*N*/);
+void opo_grids_long(const opo_derived *d, double *Tp /*SIZEL*/, double *Fp /*SIZEL*/);
+/* CW: N complex; NS: SIZEL complex */
+void opo_pump_input(const opo_derived *d, double *Ap_in);
+/* splitmix64 + Box-Muller, seeded (replaces the clock-seeded generator) */
+void opo_noise(uint64_t seed, uint64_t stream, double sigma, double *A, size_t n);
+/* Phase-matching temperature scan (functions.cuh:23-54). */
+double opo_find_pm_temp(int crystal, double lp, double ls, double Lambda, double Tmin, double Tmax,
+                        int pm_mode);
+/* Refractive index and dispersion of the crystal (ppln.cuh / spplt.cuh). */
+double opo_n(int crystal, double L, double T);
+double opo_group_velocity(int crystal, double L, double T);
+double opo_gvd(int crystal, double L, double T);
+
+/* ---------------- single-instance solver ---------------- */
+typedef struct opo_solver opo_solver;
+
+int opo_create(const opo_params *p, opo_solver **out);
+/* w, T: N doubles (angular-frequency and time grids).  Ap_in: n_ap complex
+ * (N for CW, N*NRT for NS).  Ap/As/Ai: N complex initial fields (Ai ignored
+ * for two equations, Ap may be NULL => zeros). */
+int opo_upload(opo_solver *s, const double *w, const double *T, const double *Ap_in, size_t n_ap,
+               const double *Ap, const double *As, const double *Ai);
+int opo_set_fields(opo_solver *s, const double *Ap, const double *As, const double *Ai);
+int opo_crystal_pass(opo_solver *s);
+int opo_linear_step(opo_solver *s);
+/* n_times applications of the nonlinear half step of slice `slice` (z as in
+ * the reference's accumulated z of that slice). */
+int opo_nl_half(opo_solver *s, int slice, int n_times);
+int opo_boundary(opo_solver *s);
+/* Round trips rt0 .. rt0+n_rt-1.  Saves slots per the reference when the
+ * solver has save buffers (opo_enable_save). */
+int opo_run(opo_solver *s, int rt0, int n_rt);
+int opo_enable_save(opo_solver *s, int enable);
+int opo_download(opo_solver *s, double *Ap, double *As, double *Ai);
+/* slots*N complex each (slots = Nrts for CW, NRT for NS); any may be NULL */
+int opo_download_saved(opo_solver *s, double *save_p, double *save_s, double *save_i);
+/* sum |As|^2 after each of the last run's round trips (n values) */
+int opo_power_history(opo_solver *s, double *out, int n);
+/* Selects kernel tiles: col_threads/row_threads in {64,128,256}; 0 = default.
+ * No reference counterpart (autotuner hook). */
+int opo_set_tiles(opo_solver *s, int col_threads, int row_threads);
+/* Capture one steady round trip in a hipGraph and replay it (default on).
+ * No reference counterpart. */
+int opo_use_graph(opo_solver *s, int enable);
+/* Device time of the last opo_run in milliseconds (hipEvents on the solver's
+ * stream).  No reference counterpart. */
+int opo_last_run_ms(opo_solver *s, double *ms);
+int opo_destroy(opo_solver *s);
+
+/* ---------------- batched sweeps (no single reference counterpart: the
+ * serial loop of src/cuOPO.sh:91-175 run as one launch) ---------------- */
+typedef struct opo_batch opo_batch;
+
+/* count instances; each params[k] may have its own N (grouped internally),
+ * all must share three_eqs, regime, NZ and precision. */
+int opo_batch_create(const opo_params *params, int count, opo_batch **out);
+int opo_batch_upload(opo_batch *b, int idx, const double *w, const double *T, const double *Ap_in,
+                     size_t n_ap, const double *Ap, const double *As, const double *Ai);
+/* Runs round trips until every instance has converged (relative change of
+ * sum |As|^2 over `check_every` round trips below tol) or max_rt is reached.
+ * rts_out[k] receives the round trips instance k ran (its convergence point),
+ * or -1 if it did not converge.  tol <= 0 runs exactly max_rt round trips. */
+int opo_batch_run(opo_batch *b, int max_rt, int check_every, double tol, int *rts_out);
+int opo_batch_download(opo_batch *b, int idx, double *Ap, double *As, double *Ai);
+/* final sum |As|^2 per instance (count values) */
+int opo_batch_power(opo_batch *b, double *out);
+int opo_batch_last_run_ms(opo_batch *b, double *ms);
+int opo_batch_set_tiles(opo_batch *b, int col_threads, int row_threads);
+int opo_batch_destroy(opo_batch *b);
+
+/* ---------------- errors ---------------- */
+const char *opo_strerror(int status);
+const char *opo_last_error(void);
+int opo_device_count(int *n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
