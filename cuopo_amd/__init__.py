@@ -110,6 +110,8 @@ SIGNATURES = {
     "opo_set_tiles": (_i, [_p, _i, _i]),
     "opo_use_graph": (_i, [_p, _i]),
     "opo_last_run_ms": (_i, [_p, C.POINTER(_d)]),
+    "opo_time_stages": (_i, [_p, _i, C.POINTER(_d), C.POINTER(_d)]),
+    "opo_copy_probe": (_i, [_sz, _i, _i, C.POINTER(_d), C.POINTER(_d)]),
     "opo_destroy": (_i, [_p]),
     "opo_batch_create": (_i, [C.POINTER(Params), _i, C.POINTER(_p)]),
     "opo_batch_upload": (_i, [_p, _i, _p, _p, _p, _sz, _p, _p, _p]),
@@ -327,6 +329,19 @@ class Solver:
         v = C.c_double()
         _check(lib().opo_last_run_ms(self._h, C.byref(v)))
         return v.value
+
+    def time_stages(self, iters=50):
+        """(column-stage us, row-stage us) per launch, each kernel timed alone."""
+        a, b = C.c_double(), C.c_double()
+        _check(lib().opo_time_stages(self._h, iters, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+
+def copy_probe(nbytes, iters=20, device=0):
+    """Measured streaming-copy bandwidth (GB/s, read+write) and kernel time (us)."""
+    g, u = C.c_double(), C.c_double()
+    _check(lib().opo_copy_probe(nbytes, iters, device, C.byref(g), C.byref(u)))
+    return g.value, u.value
 
 
 class Batch:

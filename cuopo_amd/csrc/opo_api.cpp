@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -169,20 +170,55 @@ int validate(const opo_params *p) {
   return OPO_OK;
 }
 
+// LDS bytes of one column-stage / row-stage block (must match the launchers)
+size_t col_lds(const Group &g, int ct) { return ((size_t)g.N1 + 2 * (size_t)g.NF * ct * g.N1) * esize(g) + 512; }
+size_t row_lds(const Group &g, int rt) { return ((size_t)g.N2 + 2 * (size_t)rt * g.N2) * esize(g); }
+constexpr size_t kMaxLds = 160 * 1024;
+
+int floor_pow2(int x) {
+  int p = 1;
+  while (p * 2 <= x) p *= 2;
+  return p;
+}
+
+// Column tile ct (columns per block, threads NF*ct*N1/8) and row tile rt (rows
+// per block, threads rt*N2/8): powers of two dividing N2 / N1, at most 256
+// threads, within the LDS of one CU.
 void pick_tiles(Engine &E, Group &g) {
   const int T1 = g.N1 / 8, T2 = g.N2 / 8;
-  int ctn = E.col_threads ? E.col_threads : (E.count == 1 ? 64 : 128);
-  int rtn = E.row_threads ? E.row_threads : (E.count == 1 ? 64 : 128);
-  g.ct = std::max(1, std::min(ctn / T1, g.N2));
-  g.rt = std::max(1, std::min(rtn / T2, g.N1));
+  // default column tile: one 128-byte line of columns (8 FP64 / 16 FP32)
+  int ct = E.col_threads ? std::max(1, E.col_threads / T1) : (int)(128 / esize(g));
+  int rtn = E.row_threads ? E.row_threads : 128;
+  ct = std::min(floor_pow2(ct), g.N2);
+  while (ct > 1 && g.NF * ct * T1 > 512) ct >>= 1;
+  g.ct = ct;
+  g.rt = std::max(1, std::min(floor_pow2(std::max(1, rtn / T2)), g.N1));
+  while (g.ct > 1 && col_lds(g, g.ct) > kMaxLds) g.ct >>= 1;
+  while (g.rt > 1 && row_lds(g, g.rt) > kMaxLds) g.rt >>= 1;
+}
+
+// host-side check of a launch shape against what the kernels assume
+int check_shapes(const Group &g) {
+  const int T1 = g.N1 / 8, T2 = g.N2 / 8;
+  if (!is_pow2(g.ct) || g.N2 % g.ct != 0 || g.NF * g.ct * T1 > 512 || col_lds(g, g.ct) > kMaxLds)
+    return fail(OPO_ERR_INVALID, "internal: column tile does not fit the kernel");
+  if (!is_pow2(g.rt) || g.N1 % g.rt != 0 || g.rt * T2 > 512 || row_lds(g, g.rt) > kMaxLds)
+    return fail(OPO_ERR_INVALID, "internal: row tile does not fit the kernel");
+  if (g.N1 < 16 || g.N2 < 16 || g.N1 > 128 || g.N2 > 4096 || (long long)g.N1 * g.N2 != g.N)
+    return fail(OPO_ERR_INVALID, "internal: four-step split out of range");
+  return OPO_OK;
 }
 
 int make_group(Engine &E, Group &g) {
   g.NF = E.three_eqs ? 3 : 2;
   g.fp32 = E.fp32;
+  // four-step split: short columns (N1 <= 64 where possible) so a column
+  // tile is a full cache line, rows up to 2048 (FP64) / 4096 (FP32) points
   const int L = ilog2ll(g.N);
-  g.N1 = 1 << (L / 2);
+  g.N1 = std::min(1 << (L / 2), 64);
   g.N2 = g.N / g.N1;
+  const int row_max = g.fp32 ? 4096 : 2048;
+  while (g.N2 > row_max) { g.N2 >>= 1; g.N1 <<= 1; }
   g.lobits = (L + 1) / 2;
   const size_t es = esize(g);
   RCHK(dalloc(g, &g.tw1, g.N1 * es));
@@ -377,7 +413,8 @@ int sync_insts(Engine &E) {
 // ------------------------------------------------------------ stage emitters
 
 int emit_col(Engine &E, Group &g, unsigned ops, int load, int inv, int fwd, int store, int s_end, int s_start,
-             int pump_off, int snap_every) {
+             int pump_off, int snap_every, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+  RCHK(check_shapes(g));
   opo::ColArgs a{};
   a.tw1 = g.tw1; a.twlo = g.twlo; a.twhi = g.twhi; a.lobits = g.lobits;
   a.N2 = g.N2; a.log2ct = ilog2ll(g.ct);
@@ -385,19 +422,20 @@ int emit_col(Engine &E, Group &g, unsigned ops, int load, int inv, int fwd, int 
   a.s_end = s_end; a.s_start = s_start; a.pump_off = pump_off; a.three_eqs = E.three_eqs;
   a.snap_every = snap_every;
   const int threads_ct = g.ct;
-  int e = opo::launch_col(g.fp32, g.N1, g.NF, threads_ct, (int)g.members.size(), g.d_insts, a, E.stream);
+  int e = opo::launch_col(g.fp32, g.N1, g.NF, threads_ct, (int)g.members.size(), g.d_insts, a, E.stream, e0, e1);
   if (e != 0) return fail(OPO_ERR_HIP, std::string("col_stage launch: ") + hipGetErrorString((hipError_t)e));
   return OPO_OK;
 }
 
-int emit_row(Engine &E, Group &g, bool gdd, int mask) {
+int emit_row(Engine &E, Group &g, bool gdd, int mask, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+  RCHK(check_shapes(g));
   opo::RowArgs a{};
   a.tw2 = g.tw2; a.N1 = g.N1; a.log2rt = ilog2ll(g.rt); a.use_gdd = gdd ? 1 : 0;
   int fl[3] = {0, 0, 0}, nf = 0;
   for (int f = 0; f < g.NF; ++f)
     if ((mask >> f) & 1) fl[nf++] = f;
   a.nf = nf; a.f0 = fl[0]; a.f1 = fl[1]; a.f2 = fl[2];
-  int e = opo::launch_row(g.fp32, g.N2, g.NF, g.rt, (int)g.members.size(), 0, g.d_insts, a, E.stream);
+  int e = opo::launch_row(g.fp32, g.N2, g.NF, g.rt, (int)g.members.size(), 0, g.d_insts, a, E.stream, e0, e1);
   if (e != 0) return fail(OPO_ERR_HIP, std::string("row_stage launch: ") + hipGetErrorString((hipError_t)e));
   return OPO_OK;
 }
@@ -537,6 +575,9 @@ int engine_init(Engine &E, const opo_params *params, int count) {
         p.device != p0.device)
       return fail(OPO_ERR_INVALID, "batch instances must share three_eqs, regime, NZ, precision and device");
   }
+  // OPO_USE_GRAPH=0 launches every stage eagerly (profilers that cannot
+  // trace graph-launched kernels); results are identical either way.
+  if (const char *ug = std::getenv("OPO_USE_GRAPH")) E.use_graph = ug[0] != '0';
   E.count = count; E.NZ = p0.NZ; E.three_eqs = p0.three_eqs ? 1 : 0; E.regime = p0.regime;
   E.fp32 = p0.precision == OPO_FP32C; E.device = p0.device;
   E.NRT = 1;
@@ -657,7 +698,8 @@ int opo_enable_save(opo_solver *s, int enable) {
 int opo_set_tiles(opo_solver *s, int col_threads, int row_threads) {
   if (!s) return fail(OPO_ERR_INVALID, "null solver");
   for (int v : {col_threads, row_threads})
-    if (v != 0 && v != 32 && v != 64 && v != 128 && v != 256) return fail(OPO_ERR_INVALID, "tile threads in {64,128,256}");
+    if (v != 0 && v != 32 && v != 64 && v != 128 && v != 256 && v != 512)
+      return fail(OPO_ERR_INVALID, "tile threads in {32,64,128,256,512}");
   s->E.col_threads = col_threads; s->E.row_threads = row_threads;
   for (auto &g : s->E.groups) pick_tiles(s->E, g);
   s->E.graph_key = -1;
@@ -781,6 +823,84 @@ int opo_power_history(opo_solver *s, double *out, int n) {
   return OPO_OK;
 }
 
+// average kernel time of the slice-interior column stage and the LIN row stage
+int engine_time_stages(Engine &E, Group &g, int iters, double *col_us, double *row_us) {
+  using namespace opo;
+  for (auto &I : g.inst)
+    if (!I.uploaded) return fail(OPO_ERR_STATE, "time_stages before upload");
+  RCHK(sync_insts(E));
+  RCHK(reset_counters(E, 0));
+  const int A = all_mask(g);
+  hipEvent_t a = nullptr, b = nullptr;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  double tc = 0, tr = 0;
+  int rc = OPO_OK;
+  for (int it = 0; it < iters + 1 && rc == OPO_OK; ++it) {  // first iteration warms up
+    float ms = 0;
+    rc = emit_col(E, g, OP_NL_END | OP_NL_START, A, A, A, A, 0, 1 % E.NZ, 0, 0, a, b);
+    if (rc != OPO_OK) break;
+    if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) { rc = fail(OPO_ERR_HIP, "event"); break; }
+    if (it) tc += ms;
+    rc = emit_row(E, g, false, A, a, b);
+    if (rc != OPO_OK) break;
+    if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) { rc = fail(OPO_ERR_HIP, "event"); break; }
+    if (it) tr += ms;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (rc != OPO_OK) return rc;
+  if (col_us) *col_us = 1e3 * tc / iters;
+  if (row_us) *row_us = 1e3 * tr / iters;
+  return OPO_OK;
+}
+
+int opo_time_stages(opo_solver *s, int iters, double *col_us, double *row_us) {
+  if (!s || iters < 1) return fail(OPO_ERR_INVALID, "bad arguments");
+  HIPCHK(hipSetDevice(s->E.device));
+  return engine_time_stages(s->E, s->E.groups[0], iters, col_us, row_us);
+}
+
+int opo_copy_probe(size_t bytes, int iters, int device, double *gbps, double *us) {
+  if (bytes < 16 || iters < 1) return fail(OPO_ERR_INVALID, "bad arguments");
+  HIPCHK(hipSetDevice(device));
+  void *x = nullptr, *y = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t a = nullptr, b = nullptr;
+  auto cleanup = [&]() {
+    if (x) (void)hipFree(x);
+    if (y) (void)hipFree(y);
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (st) (void)hipStreamDestroy(st);
+  };
+  const long long n = (long long)(bytes / 16);
+  if (hipMalloc(&x, n * 16) != hipSuccess || hipMalloc(&y, n * 16) != hipSuccess) {
+    cleanup();
+    return fail(OPO_ERR_NOMEM, "copy probe allocation");
+  }
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&a) != hipSuccess ||
+      hipEventCreate(&b) != hipSuccess || hipMemsetAsync(x, 0, n * 16, st) != hipSuccess) {
+    cleanup();
+    return fail(OPO_ERR_HIP, "copy probe setup");
+  }
+  double tot = 0;
+  for (int it = 0; it < iters + 1; ++it) {
+    float ms = 0;
+    if (opo::launch_copy(x, y, n, st, a, b) != 0 || hipEventSynchronize(b) != hipSuccess ||
+        hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+      cleanup();
+      return fail(OPO_ERR_HIP, "copy probe launch");
+    }
+    if (it) tot += ms;
+  }
+  cleanup();
+  const double sec = tot / iters * 1e-3;
+  if (gbps) *gbps = 2.0 * n * 16 / sec / 1e9;
+  if (us) *us = sec * 1e6;
+  return OPO_OK;
+}
+
 int opo_last_run_ms(opo_solver *s, double *ms) {
   if (!s || !ms) return fail(OPO_ERR_INVALID, "null");
   *ms = s->E.last_ms;
@@ -815,7 +935,8 @@ int opo_batch_upload(opo_batch *b, int idx, const double *w, const double *T, co
 int opo_batch_set_tiles(opo_batch *b, int col_threads, int row_threads) {
   if (!b) return fail(OPO_ERR_INVALID, "null batch");
   for (int v : {col_threads, row_threads})
-    if (v != 0 && v != 32 && v != 64 && v != 128 && v != 256) return fail(OPO_ERR_INVALID, "tile threads in {64,128,256}");
+    if (v != 0 && v != 32 && v != 64 && v != 128 && v != 256 && v != 512)
+      return fail(OPO_ERR_INVALID, "tile threads in {32,64,128,256,512}");
   b->E.col_threads = col_threads; b->E.row_threads = row_threads;
   for (auto &g : b->E.groups) pick_tiles(b->E, g);
   b->E.graph_key = -1;

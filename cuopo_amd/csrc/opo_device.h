@@ -72,7 +72,7 @@ template <int R, int S, typename V> __device__ __forceinline__ void dftr(V* u) {
 // v[m] holds x[t + m*M/8]; outputs go to LDS element (i/P)*P*R + k + j*P.
 // Team element e lives at buf[gbase + e*es].
 template <int M, int R, int P, int S, typename V>
-__device__ __forceinline__ void stockham_pass(const V* v, V* buf, int t, int gbase, int es, const V* __restrict__ tw) {
+__device__ __forceinline__ void stockham_pass(const V* v, V* buf, int t, int gbase, int es, const V* tw) {
   constexpr int T = M / 8;
   constexpr int NB = 8 / R;
 #pragma unroll
@@ -93,25 +93,57 @@ __device__ __forceinline__ void stockham_pass(const V* v, V* buf, int t, int gba
   }
 }
 
-// Team FFT of length M.  v[m] = x[t + m*M/8] on entry, X[t + m*M/8] on exit.
-// lds holds two ping-pong buffers `bufstride` elements apart; `ph` is a
-// block-uniform counter that alternates them across every pass of the kernel,
-// so consecutive passes never write the buffer another thread may still read.
-// Every thread of the block must call this (it contains __syncthreads).
-template <int M, int S, int Q = 0, typename V>
-__device__ __forceinline__ void team_fft(V* v, V* lds, int& ph, int t, int gbase, int es, int bufstride,
-                                         const V* __restrict__ tw) {
+// Team FFT of length M over NFX fields at once (field f transformed when bit
+// f of `mask` is set; the mask is block-uniform).  v[f][m] = x_f[t + m*M/8]
+// on entry, X_f[t + m*M/8] on exit.  Field f's element e lives at
+// lds[f*fstride + gbase + e*es]; lds holds two ping-pong buffers `bufstride`
+// elements apart, and `ph` (block-uniform) alternates them across every pass
+// of the kernel, so one barrier per pass suffices: a buffer is rewritten only
+// after the barrier that follows the next pass's writes.  `tw` points to the
+// M-point root table in LDS.  Every thread of the block must call this.
+template <int M, int S, int NFX, int Q = 0, typename V>
+__device__ __forceinline__ void team_fft(V (*v)[8], int mask, V* lds, int& ph, int t, int gbase, int es,
+                                         int fstride, int bufstride, const V* tw) {
   if constexpr (Q < npasses(M)) {
     constexpr int R = radix_at(M, Q);
     constexpr int P = p_at(Q);
     constexpr int T = M / 8;
     V* buf = lds + (ph & 1) * bufstride;
-    stockham_pass<M, R, P, S>(v, buf, t, gbase, es, tw);
+#pragma unroll
+    for (int f = 0; f < NFX; ++f)
+      if ((mask >> f) & 1) stockham_pass<M, R, P, S>(v[f], buf + f * fstride, t, gbase, es, tw);
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = buf[gbase + (t + m * T) * es];
+    for (int f = 0; f < NFX; ++f)
+      if ((mask >> f) & 1) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) v[f][m] = buf[f * fstride + gbase + (t + m * T) * es];
+      }
     ++ph;
-    team_fft<M, S, Q + 1>(v, lds, ph, t, gbase, es, bufstride, tw);
+    team_fft<M, S, NFX, Q + 1>(v, mask, lds, ph, t, gbase, es, fstride, bufstride, tw);
+  }
+}
+
+// Single-field variant: each thread owns one field (its wave's field); `on`
+// (per thread) skips the work of threads whose field is not transformed,
+// while every thread still reaches the barriers.  `buf0` is this thread's
+// field region in buffer 0; buffer 1 is `bufstride` elements further.
+template <int M, int S, int Q = 0, typename V>
+__device__ __forceinline__ void team_fft1(V* v, bool on, V* buf0, int& ph, int t, int gbase, int es, int bufstride,
+                                          const V* tw) {
+  if constexpr (Q < npasses(M)) {
+    constexpr int R = radix_at(M, Q);
+    constexpr int P = p_at(Q);
+    constexpr int T = M / 8;
+    V* buf = buf0 + (ph & 1) * bufstride;
+    if (on) stockham_pass<M, R, P, S>(v, buf, t, gbase, es, tw);
+    __syncthreads();
+    if (on) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) v[m] = buf[gbase + (t + m * T) * es];
+    }
+    ++ph;
+    team_fft1<M, S, Q + 1>(v, on, buf0, ph, t, gbase, es, bufstride, tw);
   }
 }
 

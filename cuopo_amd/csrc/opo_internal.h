@@ -85,10 +85,14 @@ struct TickArgs {
 };
 
 // Launchers (defined in opo_kernels.hip).  Return hipError_t as int.
+// ev0/ev1 (hipEvent_t, may be null): recorded at kernel start/stop through
+// hipExtLaunchKernelGGL, so their interval is the kernel alone.
 int launch_col(int fp32, int N1, int NF, int threads_ct, int count, const OpoInst *insts, const ColArgs &a,
-               void *stream);
+               void *stream, void *ev0 = nullptr, void *ev1 = nullptr);
 int launch_row(int fp32, int N2, int NF, int rt, int count, int tiles_y, const OpoInst *insts,
-               const RowArgs &a, void *stream);
+               const RowArgs &a, void *stream, void *ev0 = nullptr, void *ev1 = nullptr);
+// streaming copy of n 16-byte elements (bandwidth probe / counter calibration)
+int launch_copy(const void *src, void *dst, long long n, void *stream, void *ev0, void *ev1);
 int launch_tick(const OpoInst *insts, const TickArgs &a, void *stream);
 
 }  // namespace opo

@@ -13,6 +13,10 @@
 // Each stage reads and writes every field once with coalesced 16-byte
 // accesses; the sub-FFTs run in registers + LDS (team_fft).
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
+
+#include <algorithm>
+#include <type_traits>
 
 #include "opo_device.h"
 #include "opo_internal.h"
@@ -30,190 +34,256 @@ __device__ __forceinline__ V twiddle_n(unsigned e, const V *__restrict__ lo, con
   return tw_sign<S>(w);
 }
 
-template <typename T, int N1, int NF>
-__global__ __launch_bounds__(256) void col_stage(const OpoInst *__restrict__ insts, ColArgs a) {
+// Field f of three values.  The operands are taken by value so the choice is a
+// register select, never a load through a selected address (which would keep
+// the field arrays out of registers).
+template <typename V> __device__ __forceinline__ V pick(int f, V a, V b, V c) {
+  const V r = f == 1 ? b : c;
+  return f == 0 ? a : r;
+}
+
+// Pointwise nonlinear half step on all fields of K samples.
+template <int NF, int K, typename V>
+__device__ __forceinline__ void nl_half_pt(V *xp, V *xs, V *xi, const OpoInst *I, int s) {
+  const double *q = I->nl + 16 * s;
+  if constexpr (NF == 3) {
+    // Closed form of cwes3.cuh:153-172: every stage evaluates dAdz on the
+    // unchanged fields, so A += (h/6)(k1+2k2+2k3+k4) = A + C*P(A) with
+    // C = i kappa (h/6)(E(z)+4E(z+dz/4)+E(z+dz/2)) precomputed in FP64.
+    const V Cp = ld_c<V>(q), Cs = ld_c<V>(q + 2), Ci = ld_c<V>(q + 4);
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+      const V ap = xp[m], as = xs[m], ai = xi[m];
+      xp[m] = cadd(ap, cmul(Cp, cmul(as, ai)));
+      xs[m] = cadd(as, cmul(Cs, cmul(ap, cconj(ai))));
+      xi[m] = cadd(ai, cmul(Ci, cmul(ap, cconj(as))));
+    }
+  } else {
+    // cwes2.cuh:147-166: stages 2-4 evaluate dAdz on A + s*k (s = 0.5, 0.5, 1).
+    using R = decltype(V::x);
+    const V ep0 = ld_c<V>(q), ep1 = ld_c<V>(q + 2), ep2 = ld_c<V>(q + 4);
+    const V em0 = ld_c<V>(q + 6), em1 = ld_c<V>(q + 8), em2 = ld_c<V>(q + 10);
+    const V ikp = mk<V>((R)0, (R)I->kp), iks = mk<V>((R)0, (R)I->ks);
+    const R h = (R)I->h;
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+      const V ap = xp[m], as = xs[m];
+      V k1p = cmul(cmul(cmul(ikp, as), as), ep0), k1s = cmul(cmul(cmul(iks, ap), cconj(as)), em0);
+      V bp = cadd(ap, cscale(k1p, (R)0.5)), bs = cadd(as, cscale(k1s, (R)0.5));
+      V k2p = cmul(cmul(cmul(ikp, bs), bs), ep1), k2s = cmul(cmul(cmul(iks, bp), cconj(bs)), em1);
+      bp = cadd(ap, cscale(k2p, (R)0.5)); bs = cadd(as, cscale(k2s, (R)0.5));
+      V k3p = cmul(cmul(cmul(ikp, bs), bs), ep1), k3s = cmul(cmul(cmul(iks, bp), cconj(bs)), em1);
+      bp = cadd(ap, k3p); bs = cadd(as, k3s);
+      V k4p = cmul(cmul(cmul(ikp, bs), bs), ep2), k4s = cmul(cmul(cmul(iks, bp), cconj(bs)), em2);
+      V sp = cadd(cadd(cadd(k1p, cscale(k2p, (R)2)), cscale(k3p, (R)2)), k4p);
+      V ss = cadd(cadd(cadd(k1s, cscale(k2s, (R)2)), cscale(k3s, (R)2)), k4s);
+      sp = cscale(sp, h); ss = cscale(ss, h);
+      xp[m] = cadd(ap, mk<V>(sp.x / (R)6, sp.y / (R)6));
+      xs[m] = cadd(as, mk<V>(ss.x / (R)6, ss.y / (R)6));
+    }
+  }
+}
+
+// Column stage.  The block holds CT columns of every field; the threads are
+// split into NF groups of FT = CT*N1/8 (one wave per field at FT = 64), and
+// group f runs field f's column FFTs, so the fields' transforms proceed on
+// different SIMDs at once.  The time-domain samples of all fields meet in LDS
+// for the pointwise program, which every group evaluates for its 8 samples;
+// each group keeps its own field.  MID compiles the slice-interior variant
+// only (inverse -> NL end -> NL start -> forward).
+template <typename T, int N1, int NF, bool MID>
+__global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ insts, ColArgs a) {
   using V = typename vec2<T>::type;
   constexpr int T1 = N1 / 8;
   extern __shared__ unsigned char smem_raw[];
-  V *lds = reinterpret_cast<V *>(smem_raw);
+  V *ltw = reinterpret_cast<V *>(smem_raw);  // N1-point root table
+  V *lds = ltw + N1;                         // 2 buffers x NF fields x CT*N1
 
   const OpoInst *I = insts + blockIdx.y;
   if (*I->active == 0) return;  // block-uniform
 
   const int CT = 1 << a.log2ct;
+  const int FT = CT * T1;
   const int tid = threadIdx.x;
-  const int c = tid & (CT - 1);
-  const int t = tid >> a.log2ct;
+  const int f = tid / FT;  // this thread's field
+  const int lt = tid - f * FT;
+  const int c = lt & (CT - 1);
+  const int t = lt >> a.log2ct;
   const int N2 = a.N2;
   const int n2 = blockIdx.x * CT + c;
   const long long N = (long long)N1 * N2;
   const unsigned nmask = (unsigned)(N - 1);
   V *st = reinterpret_cast<V *>(I->state);
-  const V *tw1 = reinterpret_cast<const V *>(a.tw1);
-  const V *twlo = reinterpret_cast<const V *>(a.twlo);
-  const V *twhi = reinterpret_cast<const V *>(a.twhi);
-  const int bufstride = CT * N1;
+  const int fstride = CT * N1;
+  const int bufstride = NF * fstride;
+  const int inv_mask = MID ? (1 << NF) - 1 : a.inv_mask;
+  const int fwd_mask = MID ? (1 << NF) - 1 : a.fwd_mask;
+  const int load_mask = MID ? (1 << NF) - 1 : a.load_mask;
+  const int store_mask = MID ? (1 << NF) - 1 : a.store_mask;
+  const bool any_fft = (inv_mask | fwd_mask) != 0;
 
-  V v[NF][8];
+  if (any_fft) {
+    const V *tw1 = reinterpret_cast<const V *>(a.tw1);
+    for (int e = tid; e < N1; e += blockDim.x) ltw[e] = tw1[e];
+  }
+  V v[8];
+  const bool ld = (load_mask >> f) & 1;
 #pragma unroll
-  for (int f = 0; f < NF; ++f)
-    if ((a.load_mask >> f) & 1) {
+  for (int m = 0; m < 8; ++m) v[m] = ld ? st[f * N + (long long)N2 * (t + m * T1) + n2] : mk<V>(0, 0);
+  // four-step twiddles e^{2 pi i n2 k1 / N} for this thread's k1 = t + m*T1
+  V W[8];
+  if (any_fft) {
+    const V *twlo = reinterpret_cast<const V *>(a.twlo);
+    const V *twhi = reinterpret_cast<const V *>(a.twhi);
+    const unsigned lm = (1u << a.lobits) - 1u;
 #pragma unroll
-      for (int m = 0; m < 8; ++m) v[f][m] = st[f * N + (long long)N2 * (t + m * T1) + n2];
-    } else {
-#pragma unroll
-      for (int m = 0; m < 8; ++m) v[f][m] = mk<V>(0, 0);
+    for (int m = 0; m < 8; ++m) {
+      const unsigned e = (unsigned)(n2 * (t + m * T1)) & nmask;
+      W[m] = cmul(twhi[e >> a.lobits], twlo[e & lm]);
     }
+  }
 
   int ph = 0;
-  // ---- inverse column transform (back to time domain) ----
+  V *mine0 = lds + f * fstride;
+  // ---- inverse column transform of this group's field ----
+  const bool inv = (inv_mask >> f) & 1;
+  if (inv_mask) {
+    if (inv) {
 #pragma unroll
-  for (int f = 0; f < NF; ++f)
-    if ((a.inv_mask >> f) & 1) {
-#pragma unroll
-      for (int m = 0; m < 8; ++m)
-        v[f][m] = cmul(v[f][m], twiddle_n<-1>((unsigned)(n2 * (t + m * T1)) & nmask, twlo, twhi, a.lobits));
-      team_fft<N1, -1>(v[f], lds, ph, t, c, CT, bufstride, tw1);
+      for (int m = 0; m < 8; ++m) v[m] = cmul(v[m], cconj(W[m]));
     }
-
-  // ---- pointwise program on time samples n = N2*(t + m*T1) + n2 ----
-  const unsigned ops = a.ops;
-  auto nl_half = [&](int s) {
-    const double *q = I->nl + 16 * s;
-    if (a.three_eqs) {
-      // Closed form of cwes3.cuh:153-172: every stage evaluates dAdz on the
-      // unchanged fields, so A += (h/6)(k1+2k2+2k3+k4) = A + C*P(A) with
-      // C = i kappa (h/6)(E(z)+4E(z+dz/4)+E(z+dz/2)) precomputed in FP64.
-      const V Cp = ld_c<V>(q), Cs = ld_c<V>(q + 2), Ci = ld_c<V>(q + 4);
-#pragma unroll
-      for (int m = 0; m < 8; ++m) {
-        const V ap = v[0][m], as = v[1][m], ai = v[NF - 1][m];
-        v[0][m] = cadd(ap, cmul(Cp, cmul(as, ai)));
-        v[1][m] = cadd(as, cmul(Cs, cmul(ap, cconj(ai))));
-        if constexpr (NF == 3) v[2][m] = cadd(ai, cmul(Ci, cmul(ap, cconj(as))));
-      }
-    } else {
-      // cwes2.cuh:147-166: stages 2-4 evaluate dAdz on A + s*k (s = 0.5, 0.5, 1).
-      using R = decltype(V::x);
-      const V ep0 = ld_c<V>(q), ep1 = ld_c<V>(q + 2), ep2 = ld_c<V>(q + 4);
-      const V em0 = ld_c<V>(q + 6), em1 = ld_c<V>(q + 8), em2 = ld_c<V>(q + 10);
-      const V ikp = mk<V>((R)0, (R)I->kp), iks = mk<V>((R)0, (R)I->ks);
-      const R h = (R)I->h;
-#pragma unroll
-      for (int m = 0; m < 8; ++m) {
-        const V ap = v[0][m], as = v[1][m];
-        V k1p = cmul(cmul(cmul(ikp, as), as), ep0), k1s = cmul(cmul(cmul(iks, ap), cconj(as)), em0);
-        V bp = cadd(ap, cscale(k1p, (R)0.5)), bs = cadd(as, cscale(k1s, (R)0.5));
-        V k2p = cmul(cmul(cmul(ikp, bs), bs), ep1), k2s = cmul(cmul(cmul(iks, bp), cconj(bs)), em1);
-        bp = cadd(ap, cscale(k2p, (R)0.5)); bs = cadd(as, cscale(k2s, (R)0.5));
-        V k3p = cmul(cmul(cmul(ikp, bs), bs), ep1), k3s = cmul(cmul(cmul(iks, bp), cconj(bs)), em1);
-        bp = cadd(ap, k3p); bs = cadd(as, k3s);
-        V k4p = cmul(cmul(cmul(ikp, bs), bs), ep2), k4s = cmul(cmul(cmul(iks, bp), cconj(bs)), em2);
-        V sp = cadd(cadd(cadd(k1p, cscale(k2p, (R)2)), cscale(k3p, (R)2)), k4p);
-        V ss = cadd(cadd(cadd(k1s, cscale(k2s, (R)2)), cscale(k3s, (R)2)), k4s);
-        sp = cscale(sp, h); ss = cscale(ss, h);
-        v[0][m] = cadd(ap, mk<V>(sp.x / (R)6, sp.y / (R)6));
-        v[1][m] = cadd(as, mk<V>(ss.x / (R)6, ss.y / (R)6));
-      }
-    }
-  };
-
-  if (ops & OP_NL_END) nl_half(a.s_end);
-  if ((ops & OP_EOM) && I->eom) {
-    const V *eom = reinterpret_cast<const V *>(I->eom);
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const V e = eom[(long long)N2 * (t + m * T1) + n2];
-      v[1][m] = cmul(v[1][m], e);
-      if constexpr (NF == 3) v[2][m] = cmul(v[2][m], e);
-    }
+    team_fft1<N1, -1>(v, inv, mine0, ph, t, c, CT, bufstride, ltw);
   }
-  if (ops & OP_MIRROR) {
-    using R = decltype(V::x);
-    const V e = mk<V>((R)I->mir[0], (R)I->mir[1]);
-    const R rs = (R)I->sqrt_rs, ri = (R)I->sqrt_ri;
+  // ---- all fields meet in LDS buffer B (the last pass wrote the
+  //      transformed ones there; the others are written now) ----
+  V *xb = lds + ((ph + 1) & 1) * bufstride;
+  if (!inv) {
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      v[1][m] = cmul(cscale(v[1][m], rs), e);
-      if constexpr (NF == 3) v[2][m] = cmul(cscale(v[2][m], ri), e);
-    }
+    for (int m = 0; m < 8; ++m) xb[f * fstride + c + (t + m * T1) * CT] = v[m];
   }
-  if (ops & OP_SAVE) {
-    const int slot = *I->rt - I->save_first;
-    if (slot >= 0 && slot < I->save_slots) {
+  __syncthreads();
+  // ---- pointwise program on time samples n = N2*(t + m*T1) + n2, in two
+  //      chunks of 4 samples to bound the live registers ----
+  const unsigned ops = MID ? (OP_NL_END | OP_NL_START) : a.ops;
+  int slot = -1, rp = 0;
+  bool snap = false;
+  if constexpr (!MID) {
+    if (ops & OP_SAVE) slot = *I->rt - I->save_first;
+    if (ops & OP_PUMP) rp = *I->rt + a.pump_off;
+    snap = (ops & OP_SNAP) && a.snap_every > 0 && ((*I->rt + 1) % a.snap_every) == 0;
+  }
+  double pacc = 0;
+  // chunk of 4 samples m = hb .. hb+3 (hb a compile-time constant, so every
+  // register array below is indexed statically)
+  auto chunk = [&](auto hbc) {
+    constexpr int hb = decltype(hbc)::value;
+    V xp[4], xs[4], xi[4];  // pump, signal, idler
 #pragma unroll
-      for (int f = 0; f < NF; ++f) {
-        V *sv = reinterpret_cast<V *>(I->save[f]);
-        if (sv) {
+    for (int q = 0; q < 4; ++q) {
+      const int o = c + (t + (hb + q) * T1) * CT;
+      xp[q] = xb[o];
+      xs[q] = xb[fstride + o];
+      xi[q] = NF == 3 ? xb[2 * fstride + o] : mk<V>(0, 0);
+    }
+    if (ops & OP_NL_END) nl_half_pt<NF, 4>(xp, xs, xi, I, a.s_end);
+    if constexpr (!MID) {
+      if ((ops & OP_EOM) && I->eom) {
+        const V *eom = reinterpret_cast<const V *>(I->eom);
 #pragma unroll
-          for (int m = 0; m < 8; ++m) sv[(long long)slot * N + (long long)N2 * (t + m * T1) + n2] = v[f][m];
+        for (int q = 0; q < 4; ++q) {
+          const V e = eom[(long long)N2 * (t + (hb + q) * T1) + n2];
+          xs[q] = cmul(xs[q], e);
+          if constexpr (NF == 3) xi[q] = cmul(xi[q], e);
         }
       }
+      if (ops & OP_MIRROR) {
+        using R = decltype(V::x);
+        const V e = mk<V>((R)I->mir[0], (R)I->mir[1]);
+        const R rs = (R)I->sqrt_rs, ri = (R)I->sqrt_ri;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          xs[q] = cmul(cscale(xs[q], rs), e);
+          if constexpr (NF == 3) xi[q] = cmul(cscale(xi[q], ri), e);
+        }
+      }
+      if ((ops & OP_SAVE) || snap || (ops & OP_POWER)) {
+        V w4[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) w4[q] = pick(f, xp[q], xs[q], xi[q]);  // this group's field
+        V *sv = reinterpret_cast<V *>(I->save[f]);
+        if ((ops & OP_SAVE) && slot >= 0 && slot < I->save_slots && sv) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) sv[(long long)slot * N + (long long)N2 * (t + (hb + q) * T1) + n2] = w4[q];
+        }
+        if (snap) {
+          V *sn = reinterpret_cast<V *>(I->snap);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) sn[f * N + (long long)N2 * (t + (hb + q) * T1) + n2] = w4[q];
+        }
+        if ((ops & OP_POWER) && f == 1) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) pacc += (double)w4[q].x * (double)w4[q].x + (double)w4[q].y * (double)w4[q].y;
+        }
+      }
+      if (ops & OP_PUMP) {
+        const V *pin = reinterpret_cast<const V *>(I->pump);
+        long long off = 0;  // functions.cuh:196-210 (NS); CW copies Ap_in (cuOPO.cu:480)
+        if (I->regime == 1) {
+          if (rp == 0) off = 0;
+          else if (rp > 0 && rp < I->nrt - 1) off = (long long)rp * N;
+          else off = N * (long long)I->nrt - 1 - N;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) xp[q] = pin[off + (long long)N2 * (t + (hb + q) * T1) + n2];
+      }
+    }
+    if (ops & OP_NL_START) nl_half_pt<NF, 4>(xp, xs, xi, I, a.s_start);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[hb + q] = pick(f, xp[q], xs[q], xi[q]);
+  };
+  chunk(std::integral_constant<int, 0>{});
+  chunk(std::integral_constant<int, 4>{});
+  if constexpr (!MID) {
+    if (ops & OP_POWER) {
+      // deterministic block reduction: wave shuffle tree, then waves in order;
+      // blocks of 32 threads occupy half a wave, so start the tree inside the block
+      double acc = pacc;
+      for (int off = (blockDim.x < 64 ? (int)blockDim.x : 64) >> 1; off > 0; off >>= 1)
+        acc += __shfl_down(acc, off, 64);
+      double *red = reinterpret_cast<double *>(lds + 2 * bufstride);
+      if ((tid & 63) == 0) red[tid >> 6] = acc;
+      __syncthreads();
+      if (tid == 0) {
+        double sum = 0;
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int w = 0; w < nw; ++w) sum += red[w];
+        I->partials[blockIdx.x] = sum;
+      }
     }
   }
-  if ((ops & OP_SNAP) && a.snap_every > 0 && ((*I->rt + 1) % a.snap_every) == 0) {
-    V *sn = reinterpret_cast<V *>(I->snap);
-#pragma unroll
-    for (int f = 0; f < NF; ++f)
-#pragma unroll
-      for (int m = 0; m < 8; ++m) sn[f * N + (long long)N2 * (t + m * T1) + n2] = v[f][m];
-  }
-  if (ops & OP_POWER) {
-    double acc = 0;
-#pragma unroll
-    for (int m = 0; m < 8; ++m) acc += (double)v[1][m].x * (double)v[1][m].x + (double)v[1][m].y * (double)v[1][m].y;
-    // deterministic block reduction: wave shuffle tree, then waves in order
-    // blocks of 32 threads occupy half a wave: start the tree inside the block
-    for (int off = (blockDim.x < 64 ? (int)blockDim.x : 64) >> 1; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-    double *red = reinterpret_cast<double *>(lds + 2 * bufstride);
-    __syncthreads();
-    if ((tid & 63) == 0) red[tid >> 6] = acc;
-    __syncthreads();
-    if (tid == 0) {
-      double s = 0;
-      const int nw = (blockDim.x + 63) >> 6;
-      for (int w = 0; w < nw; ++w) s += red[w];
-      I->partials[blockIdx.x] = s;
-    }
-  }
-  if (ops & OP_PUMP) {
-    const V *pin = reinterpret_cast<const V *>(I->pump);
-    const int rp = *I->rt + a.pump_off;
-    long long off = 0;  // functions.cuh:196-210 (NS); CW copies Ap_in (cuOPO.cu:480)
-    if (I->regime == 1) {
-      if (rp == 0) off = 0;
-      else if (rp > 0 && rp < I->nrt - 1) off = (long long)rp * N;
-      else off = N * (long long)I->nrt - 1 - N;
-    }
-#pragma unroll
-    for (int m = 0; m < 8; ++m) v[0][m] = pin[off + (long long)N2 * (t + m * T1) + n2];
-  }
-  if (ops & OP_NL_START) nl_half(a.s_start);
 
-  // ---- forward column transform (start of the next linear step) ----
+  // ---- forward column transform of this group's field ----
+  const bool fw = (fwd_mask >> f) & 1;
+  if (fwd_mask) {
+    team_fft1<N1, +1>(v, fw, mine0, ph, t, c, CT, bufstride, ltw);
+    if (fw) {
 #pragma unroll
-  for (int f = 0; f < NF; ++f)
-    if ((a.fwd_mask >> f) & 1) {
-      team_fft<N1, +1>(v[f], lds, ph, t, c, CT, bufstride, tw1);
-#pragma unroll
-      for (int m = 0; m < 8; ++m)
-        v[f][m] = cmul(v[f][m], twiddle_n<+1>((unsigned)(n2 * (t + m * T1)) & nmask, twlo, twhi, a.lobits));
+      for (int m = 0; m < 8; ++m) v[m] = cmul(v[m], W[m]);
     }
-
+  }
+  if ((store_mask >> f) & 1) {
 #pragma unroll
-  for (int f = 0; f < NF; ++f)
-    if ((a.store_mask >> f) & 1) {
-#pragma unroll
-      for (int m = 0; m < 8; ++m) st[f * N + (long long)N2 * (t + m * T1) + n2] = v[f][m];
-    }
+    for (int m = 0; m < 8; ++m) st[f * N + (long long)N2 * (t + m * T1) + n2] = v[m];
+  }
 }
 
 template <typename T, int N2, int NF>
-__global__ __launch_bounds__(256) void row_stage(const OpoInst *__restrict__ insts, RowArgs a) {
+__global__ __launch_bounds__(512) void row_stage(const OpoInst *__restrict__ insts, RowArgs a) {
   using V = typename vec2<T>::type;
   constexpr int T2 = N2 / 8;
   extern __shared__ unsigned char smem_raw[];
-  V *lds = reinterpret_cast<V *>(smem_raw);
+  V *ltw = reinterpret_cast<V *>(smem_raw);  // N2-point root table
+  V *lds = ltw + N2;                         // 2 buffers x RT*N2
 
   const OpoInst *I = insts + blockIdx.y;
   if (*I->active == 0) return;
@@ -232,17 +302,22 @@ __global__ __launch_bounds__(256) void row_stage(const OpoInst *__restrict__ ins
                           : reinterpret_cast<const V *>(I->lin) + f * N;
   tab += (long long)N2 * k1;
   const V *tw2 = reinterpret_cast<const V *>(a.tw2);
+  for (int e = tid; e < N2; e += blockDim.x) ltw[e] = tw2[e];
 
-  V v[8];
+  V v[1][8];
+  V ph_tab[8];
 #pragma unroll
-  for (int m = 0; m < 8; ++m) v[m] = row[t + m * T2];
+  for (int m = 0; m < 8; ++m) v[0][m] = row[t + m * T2];
+#pragma unroll
+  for (int m = 0; m < 8; ++m) ph_tab[m] = tab[t + m * T2];
   int ph = 0;
-  team_fft<N2, +1>(v, lds, ph, t, r * N2, 1, RT * N2, tw2);
+  const int bufstride = RT * N2;
+  team_fft<N2, +1, 1>(v, 1, lds, ph, t, r * N2, 1, 0, bufstride, ltw);
 #pragma unroll
-  for (int m = 0; m < 8; ++m) v[m] = cmul(v[m], tab[t + m * T2]);
-  team_fft<N2, -1>(v, lds, ph, t, r * N2, 1, RT * N2, tw2);
+  for (int m = 0; m < 8; ++m) v[0][m] = cmul(v[0][m], ph_tab[m]);
+  team_fft<N2, -1, 1>(v, 1, lds, ph, t, r * N2, 1, 0, bufstride, ltw);
 #pragma unroll
-  for (int m = 0; m < 8; ++m) row[t + m * T2] = v[m];
+  for (int m = 0; m < 8; ++m) row[t + m * T2] = v[0][m];
 }
 
 // One thread per instance: reduce the column-tile partials in order, record
@@ -277,67 +352,106 @@ __global__ void tick_kernel(const OpoInst *__restrict__ insts, TickArgs a) {
 // ------------------------------------------------------------------ launchers
 
 template <typename T, int N1, int NF>
-static int launch_col_t(int ct, int count, const OpoInst *insts, const ColArgs &a, hipStream_t s) {
+static int launch_col_t(int ct, int count, const OpoInst *insts, const ColArgs &a, hipStream_t s, hipEvent_t e0,
+                        hipEvent_t e1) {
   using V = typename vec2<T>::type;
-  const int threads = ct * (N1 / 8);
-  const size_t lds = 2 * (size_t)ct * N1 * sizeof(V) + 64 * sizeof(double);
+  const int threads = NF * ct * (N1 / 8);
+  const size_t lds = ((size_t)N1 + 2 * (size_t)NF * ct * N1) * sizeof(V) + 64 * sizeof(double);
   dim3 grid(a.N2 / ct, count);
-  static bool big_lds = false;  // gfx950 has 160 KiB of LDS; opt in above 64 KiB once
-  if (lds > 65536 && !big_lds) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&col_stage<T, N1, NF>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    big_lds = true;
+  const bool mid = a.ops == (OP_NL_END | OP_NL_START) && a.inv_mask == (1 << NF) - 1 &&
+                   a.fwd_mask == (1 << NF) - 1 && a.load_mask == (1 << NF) - 1 && a.store_mask == (1 << NF) - 1;
+  static bool big_lds[2] = {false, false};  // gfx950 has 160 KiB of LDS; opt in above 64 KiB once
+  auto kern = mid ? &col_stage<T, N1, NF, true> : &col_stage<T, N1, NF, false>;
+  if (lds > 65536 && !big_lds[mid]) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    big_lds[mid] = true;
   }
-  hipLaunchKernelGGL((col_stage<T, N1, NF>), grid, dim3(threads), lds, s, insts, a);
+  if (e0 || e1) hipExtLaunchKernelGGL(kern, grid, dim3(threads), lds, s, e0, e1, 0, insts, a);
+  else hipLaunchKernelGGL(kern, grid, dim3(threads), lds, s, insts, a);
   return (int)hipGetLastError();
 }
 
 template <typename T, int N2, int NF>
-static int launch_row_t(int rt, int count, int tiles_y, const OpoInst *insts, const RowArgs &a, hipStream_t s) {
+static int launch_row_t(int rt, int count, int tiles_y, const OpoInst *insts, const RowArgs &a, hipStream_t s,
+                        hipEvent_t e0, hipEvent_t e1) {
   using V = typename vec2<T>::type;
   const int threads = rt * (N2 / 8);
-  const size_t lds = 2 * (size_t)rt * N2 * sizeof(V);
+  const size_t lds = ((size_t)N2 + 2 * (size_t)rt * N2) * sizeof(V);
   dim3 grid((a.N1 / rt) * a.nf, count);
   (void)tiles_y;
-  hipLaunchKernelGGL((row_stage<T, N2, NF>), grid, dim3(threads), lds, s, insts, a);
+  static bool big_lds = false;
+  if (lds > 65536 && !big_lds) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&row_stage<T, N2, NF>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    big_lds = true;
+  }
+  if (e0 || e1) hipExtLaunchKernelGGL((row_stage<T, N2, NF>), grid, dim3(threads), lds, s, e0, e1, 0, insts, a);
+  else hipLaunchKernelGGL((row_stage<T, N2, NF>), grid, dim3(threads), lds, s, insts, a);
   return (int)hipGetLastError();
 }
 
-#define OPO_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512)
+#define OPO_COL_SIZES(X) X(16) X(32) X(64) X(128)
+#define OPO_ROW_SIZES(X) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
 
 template <typename T, int NF>
-static int dispatch_col(int N1, int ct, int count, const OpoInst *insts, const ColArgs &a, hipStream_t s) {
+static int dispatch_col(int N1, int ct, int count, const OpoInst *insts, const ColArgs &a, hipStream_t s,
+                        hipEvent_t e0, hipEvent_t e1) {
   switch (N1) {
 #define CASE(M) \
-  case M: return launch_col_t<T, M, NF>(ct, count, insts, a, s);
-    OPO_SIZES(CASE)
+  case M: return launch_col_t<T, M, NF>(ct, count, insts, a, s, e0, e1);
+    OPO_COL_SIZES(CASE)
 #undef CASE
   }
   return (int)hipErrorInvalidValue;
 }
 
 template <typename T, int NF>
-static int dispatch_row(int N2, int rt, int count, int ty, const OpoInst *insts, const RowArgs &a, hipStream_t s) {
+static int dispatch_row(int N2, int rt, int count, int ty, const OpoInst *insts, const RowArgs &a, hipStream_t s,
+                        hipEvent_t e0, hipEvent_t e1) {
   switch (N2) {
 #define CASE(M) \
-  case M: return launch_row_t<T, M, NF>(rt, count, ty, insts, a, s);
-    OPO_SIZES(CASE)
+  case M: return launch_row_t<T, M, NF>(rt, count, ty, insts, a, s, e0, e1);
+    OPO_ROW_SIZES(CASE)
 #undef CASE
   }
   return (int)hipErrorInvalidValue;
 }
 
-int launch_col(int fp32, int N1, int NF, int ct, int count, const OpoInst *insts, const ColArgs &a, void *stream) {
+int launch_col(int fp32, int N1, int NF, int ct, int count, const OpoInst *insts, const ColArgs &a, void *stream,
+               void *ev0, void *ev1) {
   hipStream_t s = (hipStream_t)stream;
-  if (fp32) return NF == 3 ? dispatch_col<float, 3>(N1, ct, count, insts, a, s) : dispatch_col<float, 2>(N1, ct, count, insts, a, s);
-  return NF == 3 ? dispatch_col<double, 3>(N1, ct, count, insts, a, s) : dispatch_col<double, 2>(N1, ct, count, insts, a, s);
+  hipEvent_t e0 = (hipEvent_t)ev0, e1 = (hipEvent_t)ev1;
+  if (fp32)
+    return NF == 3 ? dispatch_col<float, 3>(N1, ct, count, insts, a, s, e0, e1)
+                   : dispatch_col<float, 2>(N1, ct, count, insts, a, s, e0, e1);
+  return NF == 3 ? dispatch_col<double, 3>(N1, ct, count, insts, a, s, e0, e1)
+                 : dispatch_col<double, 2>(N1, ct, count, insts, a, s, e0, e1);
 }
 
 int launch_row(int fp32, int N2, int NF, int rt, int count, int ty, const OpoInst *insts, const RowArgs &a,
-               void *stream) {
+               void *stream, void *ev0, void *ev1) {
   hipStream_t s = (hipStream_t)stream;
-  if (fp32) return NF == 3 ? dispatch_row<float, 3>(N2, rt, count, ty, insts, a, s) : dispatch_row<float, 2>(N2, rt, count, ty, insts, a, s);
-  return NF == 3 ? dispatch_row<double, 3>(N2, rt, count, ty, insts, a, s) : dispatch_row<double, 2>(N2, rt, count, ty, insts, a, s);
+  hipEvent_t e0 = (hipEvent_t)ev0, e1 = (hipEvent_t)ev1;
+  if (fp32)
+    return NF == 3 ? dispatch_row<float, 3>(N2, rt, count, ty, insts, a, s, e0, e1)
+                   : dispatch_row<float, 2>(N2, rt, count, ty, insts, a, s, e0, e1);
+  return NF == 3 ? dispatch_row<double, 3>(N2, rt, count, ty, insts, a, s, e0, e1)
+                 : dispatch_row<double, 2>(N2, rt, count, ty, insts, a, s, e0, e1);
+}
+
+// plain streaming copy, 16 bytes per thread per iteration (grid-stride)
+__global__ __launch_bounds__(256) void copy_kernel(const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                                    long long n) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+int launch_copy(const void *src, void *dst, long long n, void *stream, void *ev0, void *ev1) {
+  const long long blocks = std::min<long long>((n + 255) / 256, 256LL * 64);
+  hipExtLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, (hipEvent_t)ev0,
+                        (hipEvent_t)ev1, 0, (const double2 *)src, (double2 *)dst, n);
+  return (int)hipGetLastError();
 }
 
 int launch_tick(const OpoInst *insts, const TickArgs &a, void *stream) {

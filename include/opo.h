@@ -138,7 +138,8 @@ int opo_download(opo_solver *s, double *Ap, double *As, double *Ai);
 int opo_download_saved(opo_solver *s, double *save_p, double *save_s, double *save_i);
 /* sum |As|^2 after each of the last run's round trips (n values) */
 int opo_power_history(opo_solver *s, double *out, int n);
-/* Selects kernel tiles: col_threads/row_threads in {64,128,256}; 0 = default.
+/* Selects kernel tiles: col_threads (per field) / row_threads in
+ * {32,64,128,256,512}; 0 = default (128-byte column tiles, 128-thread rows).
  * No reference counterpart (autotuner hook). */
 int opo_set_tiles(opo_solver *s, int col_threads, int row_threads);
 /* Capture one steady round trip in a hipGraph and replay it (default on).
@@ -147,6 +148,11 @@ int opo_use_graph(opo_solver *s, int enable);
 /* Device time of the last opo_run in milliseconds (hipEvents on the solver's
  * stream).  No reference counterpart. */
 int opo_last_run_ms(opo_solver *s, double *ms);
+/* Measurement hook: average device time (microseconds) of one column-stage
+ * and one row-stage launch of a crystal slice, each timed alone with
+ * start/stop events on the solver's stream (iters launches each).  Leaves the
+ * fields undefined.  No reference counterpart. */
+int opo_time_stages(opo_solver *s, int iters, double *col_us, double *row_us);
 int opo_destroy(opo_solver *s);
 
 /* ---------------- batched sweeps (no single reference counterpart: the
@@ -171,6 +177,11 @@ int opo_batch_set_tiles(opo_batch *b, int col_threads, int row_threads);
 int opo_batch_destroy(opo_batch *b);
 
 /* ---------------- errors ---------------- */
+/* Streaming copy of `bytes` between two device buffers, iters times:
+ * measured bandwidth (GB/s, read+write bytes) and average kernel time.  Used
+ * as the measured bandwidth peak and to scale HBM counters.  No reference
+ * counterpart. */
+int opo_copy_probe(size_t bytes, int iters, int device, double *gbps, double *us);
 const char *opo_strerror(int status);
 const char *opo_last_error(void);
 int opo_device_count(int *n);
