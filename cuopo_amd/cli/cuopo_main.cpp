@@ -1,0 +1,279 @@
+// cuopo_main.cpp -- drop-in replacement for the cuOPO executable
+// (src/cuOPO.cu:52-688): the same compile-time switches, the same 12 positional
+// arguments, the same stdout parameter dump and the same .dat outputs, with
+// the round-trip loop running on one MI355X through libcuopo_amd.so.
+//
+//   hipcc cuopo_main.cpp -D<CW_OPO|NS_OPO> -D<PPLN|SPPLT> [-DTHREE_EQS] -lcuopo_amd -o cuOPO
+//   ./cuOPO <LP_nm> <T_C> <Lambda_um> <N> <R_%> <delta_pi> <GDD_%> <TOD_%> <UPM> <beta_pi> <df> <Lcr_mm> [flags]
+//
+// Optional trailing flags (defaults reproduce the reference):
+//   --N=<pow2> --nz=<n> --nrt=<n>     grid size, slices per pass, round trips
+//   --seed=<u64> --noise=<sigma>      seeded Gaussian noise on signal/idler
+//   --ls=<um>                         signal wavelength (reference: 2*lp)
+//   --pm=3wave                        phase-match the three-wave mismatch
+//   --out=<dir>                       output directory (default: cwd)
+//   --fp32c                           FP32-complex variant
+//   --no-inputs                       skip the *_input_*.dat files
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "../../include/opo.h"
+
+#if !defined(CW_OPO) && !defined(NS_OPO)
+#error "define the regime: -DCW_OPO or -DNS_OPO (src/cuOPO.cu:65-70)"
+#endif
+#if !defined(PPLN) && !defined(SPPLT)
+#error "define the crystal: -DPPLN or -DSPPLT (src/cuOPO.cu:36-41)"
+#endif
+
+#ifndef OPO_SIZE
+#define OPO_SIZE (1 << 14)  // SIZE, src/cuOPO.cu:28
+#endif
+#ifndef OPO_NZ
+#define OPO_NZ 150          // NZ, src/cuOPO.cu:29
+#endif
+#ifndef OPO_NRT
+#define OPO_NRT 10000       // NRT, src/cuOPO.cu:30
+#endif
+
+// CHECK() of src/headers/common.cuh:6-16: print and exit(1)
+#define CHECK(call)                                                                      \
+  do {                                                                                   \
+    const int st_ = (call);                                                              \
+    if (st_ != OPO_OK) {                                                                 \
+      std::fprintf(stderr, "Error: %s:%d, ", __FILE__, __LINE__);                        \
+      std::fprintf(stderr, "code: %d, reason: %s (%s)\n", st_, opo_strerror(st_), opo_last_error()); \
+      std::exit(1);                                                                      \
+    }                                                                                    \
+  } while (0)
+
+namespace {
+
+double seconds() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// One value per line at 20 significant digits (SaveVectorReal/Complex,
+// src/headers/files.cuh:21-62); complex arrays split into _r/_i files.
+bool save_real(const std::string &path, const double *v, size_t n) {
+  FILE *f = std::fopen(path.c_str(), "w");
+  if (!f) return false;
+  std::vector<char> buf(1 << 20);
+  std::setvbuf(f, buf.data(), _IOFBF, buf.size());
+  for (size_t i = 0; i < n; ++i) std::fprintf(f, "%.20g\n", v[i]);
+  std::fclose(f);
+  return true;
+}
+bool save_complex(const std::string &stem, const double *v, size_t n) {
+  std::vector<double> re(n), im(n);
+  for (size_t i = 0; i < n; ++i) { re[i] = v[2 * i]; im[i] = v[2 * i + 1]; }
+  return save_real(stem + "_r.dat", re.data(), n) && save_real(stem + "_i.dat", im.data(), n);
+}
+
+void usage(const char *prog) {
+  std::fprintf(stderr,
+               "usage: %s <LP_nm> <T_C> <Lambda_um> <N> <R_%%> <delta_pi> <GDD_%%> <TOD_%%> <UPM> <beta_pi> <df> "
+               "<Lcr_mm> [--N=] [--nz=] [--nrt=] [--seed=] [--noise=] [--ls=] [--pm=3wave] [--out=] [--fp32c] "
+               "[--no-inputs]\n",
+               prog);
+}
+
+}  // namespace
+
+int main(int argc, char *argv[]) {
+  std::cout << "\n\n\n#######---Welcome to OPO simulator---#######\n\n\n" << std::endl;
+  time_t current_time;
+  time(&current_time);
+  std::cout << "Date: " << ctime(&current_time) << std::endl;
+  const double iStart = seconds();
+#ifdef CW_OPO
+  std::cout << "Regime: continuous wave" << std::endl;
+#endif
+#ifdef NS_OPO
+  std::cout << "Regime: nanosecond" << std::endl;
+#endif
+#ifdef THREE_EQS
+  std::cout << "Three equations" << std::endl;
+#else
+  std::cout << "Two equations" << std::endl;
+#endif
+  if (argc < 13) {
+    usage(argv[0]);
+    return 1;
+  }
+
+  opo_args a{};
+#ifdef THREE_EQS
+  a.three_eqs = 1;
+#endif
+#ifdef NS_OPO
+  a.regime = OPO_NS;
+#else
+  a.regime = OPO_CW;
+#endif
+#ifdef SPPLT
+  a.crystal = OPO_SPPLT;
+#else
+  a.crystal = OPO_PPLN;
+#endif
+  a.N = OPO_SIZE; a.NZ = OPO_NZ; a.NRT = OPO_NRT;
+  a.lp_nm = std::atof(argv[1]); a.temp_arg = std::atof(argv[2]); a.Lambda = std::atof(argv[3]);
+  a.Nth = std::atof(argv[4]); a.R_pct = std::atof(argv[5]); a.delta = std::atof(argv[6]);
+  a.gdd_pct = std::atof(argv[7]); a.tod_pct = std::atof(argv[8]); a.upm = std::atoi(argv[9]);
+  a.md = std::atof(argv[10]); a.fm = std::atof(argv[11]); a.lcr_mm = std::atof(argv[12]);
+  uint64_t seed = (uint64_t)std::chrono::system_clock::now().time_since_epoch().count();  // functions.cuh:60
+  double sigma = 1.0e-15;                                                                    // functions.cuh:62
+  std::string out = ".";
+  int precision = OPO_FP64;
+  bool save_inputs = true;
+  for (int k = 13; k < argc; ++k) {
+    const std::string f = argv[k];
+    auto val = [&](const char *key) -> const char * {
+      const size_t n = std::strlen(key);
+      return f.compare(0, n, key) == 0 ? f.c_str() + n : nullptr;
+    };
+    if (const char *v = val("--N=")) a.N = std::atoi(v);
+    else if (const char *v = val("--nz=")) a.NZ = std::atoi(v);
+    else if (const char *v = val("--nrt=")) a.NRT = std::atoi(v);
+    else if (const char *v = val("--seed=")) seed = std::strtoull(v, nullptr, 10);
+    else if (const char *v = val("--noise=")) sigma = std::atof(v);
+    else if (const char *v = val("--ls=")) a.ls_um = std::atof(v);
+    else if (const char *v = val("--out=")) out = v;
+    else if (f == "--pm=3wave") a.pm_mode = OPO_PM_THREE_WAVE;
+    else if (f == "--fp32c") precision = OPO_FP32C;
+    else if (f == "--no-inputs") save_inputs = false;
+    else { usage(argv[0]); return 1; }
+  }
+
+  opo_derived d{};
+  CHECK(opo_derive(&a, &d));
+  const size_t N = (size_t)d.N;
+  std::vector<double> T(N), w(N), Tp(d.SIZEL), Fp(d.SIZEL);
+  opo_grids(&d, T.data(), w.data());
+  opo_grids_long(&d, Tp.data(), Fp.data());
+  const size_t n_ap = d.regime == OPO_CW ? N : d.SIZEL;
+  std::vector<double> Ap_in(2 * n_ap), As(2 * N), Ai(2 * N, 0.0);
+  opo_pump_input(&d, Ap_in.data());
+  opo_noise(seed, 0, sigma, As.data(), N);
+  if (d.three_eqs) opo_noise(seed, 1, sigma, Ai.data(), N);
+
+  if (save_inputs) {  // src/cuOPO.cu:272-286
+    save_complex(out + "/pump_input", Ap_in.data(), n_ap);
+    save_complex(out + "/signal_input", As.data(), N);
+    if (d.three_eqs) save_complex(out + "/idler_input", Ai.data(), N);
+  }
+
+  // parameter dump, src/cuOPO.cu:289-369
+  std::cout << "\n\nSimulation parameters:\n\n " << std::endl;
+  std::cout << "Number of round trips   = " << d.NRT << std::endl;
+  std::cout << "Pump wavelength         = " << d.lp * 1e3 << " nm" << std::endl;
+  std::cout << "Signal wavelength       = " << d.ls * 1e3 << " nm" << std::endl;
+  if (d.three_eqs) std::cout << "Idler wavelength        = " << d.li * 1e3 << " nm" << std::endl;
+  std::cout << "Temp                    = " << d.Temp << " ºC" << std::endl;
+  std::cout << "np                      = " << d.np << std::endl;
+  std::cout << "ns                      = " << d.ns << std::endl;
+  if (d.three_eqs) std::cout << "ni                      = " << d.ni << std::endl;
+  std::cout << "ν⁻¹ pump                = " << 1.0 / d.vp << " ps/μm" << std::endl;
+  std::cout << "ν⁻¹ signal              = " << 1.0 / d.vs << " ps/μm" << std::endl;
+  if (d.three_eqs) std::cout << "ν⁻¹ idler               = " << 1.0 / d.vi << " ps/μm" << std::endl;
+  std::cout << "Δk                      = " << d.dk << " μm⁻¹" << std::endl;
+  std::cout << "Δk'                     = " << d.dkp << " ps/μm" << std::endl;
+  std::cout << "GVD pump                = " << d.b2p << " ps²/μm" << std::endl;
+  std::cout << "GVD signal              = " << d.b2s << " ps²/μm" << std::endl;
+  if (d.three_eqs) std::cout << "GVD idler               = " << d.b2i << " ps²/μm" << std::endl;
+  std::cout << "GVD compensation        = " << std::atoi(argv[7]) << " %" << std::endl;
+  std::cout << "Net GVD                 = " << (1 - d.epsilon) * d.b2s << " ps²/μm" << std::endl;
+  std::cout << "TOD pump                = " << d.b3p << " ps³/μm" << std::endl;
+  std::cout << "TOD signal              = " << d.b3s << " ps³/μm" << std::endl;
+  std::cout << "TOD compensation        = " << std::atof(argv[8]) << " %" << std::endl;
+  std::cout << "Net TOD                 = " << (1 - 0.01 * std::atoi(argv[8])) * d.b3s << " ps³/μm" << std::endl;
+  std::cout << "dQ                      = " << d.dQ * 1e6 << " pm/V" << std::endl;
+  std::cout << "Λ                       = " << d.Lambda << " μm" << std::endl;
+  std::cout << "αcp                     = " << d.alpha_crp << " μm⁻¹" << std::endl;
+  std::cout << "αcs                     = " << d.alpha_crs << " μm⁻¹" << std::endl;
+  std::cout << "αs                      = " << d.alphas << std::endl;
+  if (d.three_eqs) {
+    std::cout << "αci                      = " << d.alpha_cri << " μm⁻¹" << std::endl;
+    std::cout << "αi                      = " << d.alphai << std::endl;
+  }
+  std::cout << "Crystal length          = " << d.Lcr * 1e-3 << " mm" << std::endl;
+  std::cout << "Cavity  length          = " << d.Lcav * 1e-3 << " mm" << std::endl;
+  std::cout << "Δz                      = " << d.dz << " μm" << std::endl;
+  std::cout << "Reflectivity (signal)   = " << d.Rs * 100 << " %" << std::endl;
+  if (d.three_eqs) std::cout << "Reflectivity (idler)    = " << d.Ri * 100 << " %" << std::endl;
+  std::cout << "Cavity Finesse          = " << d.finesse << std::endl;
+  std::cout << "Cavity lw (FWHM)        = " << d.lw << " MHz" << std::endl;
+  std::cout.precision(15);
+  std::cout << "Round-trip time         = " << d.t_rt << " ps" << std::endl;
+  std::cout << "FSR                     = " << d.FSR * 1e3 << " GHz" << std::endl;
+  std::cout << "Cavity detuning (δ)     = " << d.delta << "π" << std::endl;
+  std::cout << "Using N                 = " << d.N << " points" << std::endl;
+  std::cout << "dT                      = " << d.dT << " ps" << std::endl;
+  std::cout << "SIZEL                   = " << d.SIZEL << std::endl;
+  std::cout << "dFp                     = " << d.dFp << " THz" << std::endl;
+  std::cout << "Max frequency           = " << Fp[d.SIZEL - 1] << " THz" << std::endl;
+  std::cout << "Ap0                     = " << d.Ap0 << " V/um" << std::endl;
+  std::cout << "waist                   = " << d.waist << " μm" << std::endl;
+  std::cout << "spot                    = " << d.spot << " μm²" << std::endl;
+  std::cout << "Power threshold         = " << d.Pth << " W" << std::endl;
+  std::cout << "Power                   = " << d.Power << " W" << std::endl;
+  std::cout << "Times above the thres.  = " << d.Nth << std::endl;
+  std::cout << "(ΓL)²                   = " << d.GammaL * d.GammaL << std::endl;
+  if (d.use_eom) {
+    std::cout << "Using a phase modulator:" << std::endl;
+    std::cout << "Mod. depth (β)          = " << a.md << "π rad = " << d.mod_depth << " rad" << std::endl;
+    std::cout << "Freq. detuning (δf)     = " << d.df * 1e6 << " MHz" << std::endl;
+    std::cout << "Mod. frequency(fm)      = " << d.fpm * 1e3 << " GHz" << std::endl;
+  } else {
+    std::cout << "No phase modulator" << std::endl;
+  }
+
+  // solver (device allocation + uploads, src/cuOPO.cu:379-442)
+  opo_params p{};
+  opo_params_from_derived(&d, precision, &p);
+  opo_solver *s = nullptr;
+  CHECK(opo_create(&p, &s));
+  std::vector<double> Ap(Ap_in.begin(), Ap_in.begin() + 2 * N);
+  CHECK(opo_upload(s, w.data(), T.data(), Ap_in.data(), n_ap, Ap.data(), As.data(), d.three_eqs ? Ai.data() : nullptr));
+  CHECK(opo_enable_save(s, 1));
+  std::cout << "Starting main loop on CPU & GPU...\n" << std::endl;
+
+  // round-trip loop (src/cuOPO.cu:474-610) in chunks of 500 for the progress line
+  for (int nn = 0; nn < d.NRT;) {
+    std::cout << "#round trip: " << nn << " - Completed " << (long long)nn * 100 / d.NRT << "%" << "\t\r" << std::flush;
+    const int chunk = std::min(500, d.NRT - nn);
+    CHECK(opo_run(s, nn, chunk));
+    nn += chunk;
+  }
+  std::cout << "#round trip: " << d.NRT - 1 << " - Completed " << (long long)(d.NRT - 1) * 100 / d.NRT << "%" << std::endl;
+
+  // outputs, src/cuOPO.cu:620-635
+  std::cout << "\nSaving time and frequency vectors...\n" << std::endl;
+  save_real(out + "/Tp.dat", Tp.data(), d.SIZEL);
+  save_real(out + "/freq.dat", Fp.data(), d.SIZEL);
+  save_real(out + "/T.dat", T.data(), N);
+  std::vector<double> sp(2 * d.SIZEL), ss(2 * d.SIZEL), si(d.three_eqs ? 2 * d.SIZEL : 0);
+  CHECK(opo_download_saved(s, sp.data(), ss.data(), d.three_eqs ? si.data() : nullptr));
+  save_complex(out + "/signal_output", ss.data(), d.SIZEL);
+  save_complex(out + "/pump_output", sp.data(), d.SIZEL);
+  if (d.three_eqs) save_complex(out + "/idler_output", si.data(), d.SIZEL);
+  CHECK(opo_destroy(s));
+
+  const double iElaps = seconds() - iStart;
+  if (iElaps > 60) std::cout << "\n\n...time elapsed " << iElaps / 60.0 << " min\n\n " << std::endl;
+  else std::cout << "\n\n...time elapsed " << iElaps << " seconds\n\n " << std::endl;
+  time(&current_time);
+  std::cout << ctime(&current_time) << std::endl;
+  return 0;
+}

@@ -1,0 +1,84 @@
+"""The drop-in cuOPO executables: argument contract, parameter dump, error
+convention (CPU) and the .dat outputs against the oracle (GPU)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.conftest import ROOT
+from tests.helpers import rel_l2
+
+CLI = os.path.join(ROOT, "cuopo_amd", "cli")
+
+
+def exe(name):
+    path = os.path.join(CLI, name)
+    if not os.path.exists(path):
+        pytest.fail(f"{path} not built (make -C cuopo_amd/cli)")
+    return path
+
+
+def run(args, cwd):
+    return subprocess.run(args, cwd=cwd, capture_output=True, text=True, timeout=600)
+
+
+def test_usage_on_missing_arguments(tmp_path):
+    r = run([exe("cuOPO_cw_ppln_2eq"), "532", "0", "6.97"], tmp_path)
+    assert r.returncode == 1
+    assert "usage:" in r.stderr
+
+
+def test_parameter_dump_and_error_convention(tmp_path):
+    """The parameter dump carries the setup scalars; a rejected grid exits 1
+    with the reference's 'Error: file:line, code:, reason:' line."""
+    r = run([exe("cuOPO_cw_spplt_3eq"), "532", "0", "7.97", "3", "98", "0.1", "0", "0", "0", "0", "0", "20",
+             "--N=1000", "--ls=1.0", "--pm=3wave", "--no-inputs"], tmp_path)
+    assert r.returncode == 1
+    assert "Three equations" in r.stdout and "Regime: continuous wave" in r.stdout
+    assert "Error: " in r.stderr and "code: -1" in r.stderr
+    cfg = O.Config(three_eqs=True, regime=O.CW, crystal=O.SPPLT, N=1000, NZ=150, NRT=10000, lp_nm=532,
+                   Lambda=7.97, Nth=3, R_pct=98, delta=0.1, lcr_mm=20, ls_um=1.0, pm_mode=O.PM_THREE_WAVE)
+    p = O.setup(cfg)
+    line = [l for l in r.stdout.splitlines() if l.startswith("Power threshold")][0]
+    assert float(line.split("=")[1].split()[0]) == pytest.approx(p.Pth, rel=1e-5)
+    line = [l for l in r.stdout.splitlines() if l.startswith("Temp ")][0]
+    assert float(line.split("=")[1].split()[0]) == pytest.approx(p.Temp, rel=1e-5)
+
+
+def read_c(stem):
+    return np.loadtxt(stem + "_r.dat") + 1j * np.loadtxt(stem + "_i.dat")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant", ["cuOPO_cw_ppln_3eq", "cuOPO_ns_ppln_3eq"])
+def test_cli_outputs_match_oracle(gpu, tmp_path, variant):
+    ns = "_ns_" in variant
+    N, NZ, NRT = 1024, 4, (8 if ns else 70)
+    argv = ["532", "0", "6.97", "4", "98", "0.05", "100" if ns else "0", "0", "1" if ns else "0", "0.25", "4", "5"]
+    r = run([exe(variant)] + argv + [f"--N={N}", f"--nz={NZ}", f"--nrt={NRT}", "--seed=3", "--noise=1e-6"],
+            tmp_path)
+    assert r.returncode == 0, r.stderr[-2000:]
+    cfg = O.Config(three_eqs=True, regime=O.NS if ns else O.CW, crystal=O.PPLN, N=N, NZ=NZ, NRT=NRT, lp_nm=532,
+                   Lambda=6.97, Nth=4, R_pct=98, delta=0.05, gdd_pct=100 if ns else 0, upm=1 if ns else 0,
+                   md=0.25, fm=4, lcr_mm=5)
+    p = O.setup(cfg)
+    T, w = O.grids(p)
+    Ap_in = O.pump_input(p)
+    As = read_c(str(tmp_path / "signal_input"))
+    Ai = read_c(str(tmp_path / "idler_input"))
+    assert rel_l2(As, O.noise(3, 0, 1e-6, N)) < 1e-15
+    assert rel_l2(read_c(str(tmp_path / "pump_input")), Ap_in) < 1e-15
+    Ap = Ap_in[:N].copy()
+    sv, _ = O.run(p, w, T, Ap_in, Ap, As, Ai, 0, NRT, save=True)
+    slots = NRT if ns else 64
+    for stem, key in (("signal_output", "s"), ("pump_output", "p"), ("idler_output", "i")):
+        out = read_c(str(tmp_path / stem))
+        assert out.size == slots * N
+        assert rel_l2(out, sv[key].reshape(-1)) < 1e-9, stem
+    Tp = np.loadtxt(tmp_path / "Tp.dat")
+    Tpo, Fpo = O.grids_long(p)
+    assert Tp.size == p.SIZEL and np.allclose(Tp, Tpo, rtol=1e-15)
+    assert np.allclose(np.loadtxt(tmp_path / "T.dat"), T, rtol=1e-15)
+    assert np.allclose(np.loadtxt(tmp_path / "freq.dat"), Fpo, rtol=1e-15)

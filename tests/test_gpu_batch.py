@@ -1,0 +1,191 @@
+"""Batched sweeps (instances of several grid sizes in one launch per stage),
+the convergence test with snapshots, the FP32-complex variant's error against
+the FP64 oracle, and failure handling.  GPU tests through the C ABI."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import c1, c2, c5, inputs, rel_l2, to_args
+
+pytestmark = pytest.mark.gpu
+
+
+def instance(P, cfg, seed, sigma, precision=0):
+    d = P.derive(to_args(cfg))
+    p = P.params_from(d, precision)
+    T, w = P.grids(d)
+    Ap_in = P.pump_input(d)
+    As = O.noise(seed, 0, sigma, d.N)
+    Ai = O.noise(seed, 1, sigma, d.N) if cfg.three_eqs else None
+    return p, dict(w=w, T=T, Ap_in=Ap_in, Ap=Ap_in[: d.N].copy(), As=As, Ai=Ai)
+
+
+def sweep_configs(n=6):
+    rng = np.random.default_rng(0)
+    out = []
+    for k in range(n):
+        cfg = c2(N=[1024, 4096, 2048][k % 3], NZ=6, NRT=200)
+        cfg.Nth = float(rng.uniform(1.2, 5.0)) * 3
+        cfg.delta = float(rng.uniform(-np.pi, np.pi)) / np.pi
+        cfg.R_pct = float(rng.uniform(90, 99))
+        out.append(cfg)
+    return out
+
+
+def test_batch_equals_single_instances(gpu):
+    P = gpu
+    cfgs = sweep_configs(6)
+    made = [instance(P, c, 100 + k, 1e-4) for k, c in enumerate(cfgs)]
+    b = P.Batch([m[0] for m in made])
+    for k, (p, x) in enumerate(made):
+        b.upload(k, x["w"], x["T"], x["Ap_in"], x["Ap"], x["As"], x["Ai"])
+    rts = b.run(40, check_every=0, tol=0.0)
+    assert np.all(rts == -1)
+    for k, (p, x) in enumerate(made):
+        s = P.Solver(p)
+        s.upload(x["w"], x["T"], x["Ap_in"], x["Ap"], x["As"], x["Ai"])
+        s.run(0, 40)
+        for a, c in zip(b.download(k), s.download()):
+            assert rel_l2(a, c) < 1e-14, k
+
+
+def test_batch_matches_oracle(gpu):
+    P = gpu
+    cfgs = sweep_configs(3)
+    made = [instance(P, c, 7 + k, 1e-3) for k, c in enumerate(cfgs)]
+    b = P.Batch([m[0] for m in made])
+    for k, (p, x) in enumerate(made):
+        b.upload(k, x["w"], x["T"], x["Ap_in"], x["Ap"], x["As"], x["Ai"])
+    b.run(5, check_every=0, tol=0.0)
+    for k, cfg in enumerate(cfgs):
+        po = O.setup(cfg)
+        T, w = O.grids(po)
+        x = made[k][1]
+        Ap, As, Ai = x["Ap"].copy(), x["As"].copy(), x["Ai"].copy()
+        O.run(po, w, T, x["Ap_in"], Ap, As, Ai, 0, 5)
+        gp, gs, gi = b.download(k)
+        assert rel_l2(gs, As) < 1e-9 and rel_l2(gi, Ai) < 1e-9 and rel_l2(gp, Ap) < 1e-9
+
+
+def test_convergence_snapshot_is_the_converged_state(gpu):
+    """An instance declared converged at round trip r (relative power change
+    over check_every round trips below tol) is frozen; its downloaded fields
+    equal a single-instance run of exactly r round trips.  A below-threshold
+    instance in the same batch never converges."""
+    P = gpu
+    cfg = c2(N=1024, NZ=8, NRT=3000)
+    cfg.Nth = 10.0
+    p, x = instance(P, cfg, 5, 1e-6)
+    cfg2 = c2(N=1024, NZ=8, NRT=3000)
+    cfg2.Nth = 0.5  # below threshold: decays, never converges
+    p2, x2 = instance(P, cfg2, 6, 1e-6)
+    b = P.Batch([p, p2])
+    b.upload(0, x["w"], x["T"], x["Ap_in"], x["Ap"], x["As"], x["Ai"])
+    b.upload(1, x2["w"], x2["T"], x2["Ap_in"], x2["Ap"], x2["As"], x2["Ai"])
+    rts = b.run(3000, check_every=50, tol=1e-3)
+    assert rts[0] > 0 and rts[0] % 50 == 0
+    assert rts[1] == -1
+    s = P.Solver(p)
+    s.upload(x["w"], x["T"], x["Ap_in"], x["Ap"], x["As"], x["Ai"])
+    s.run(0, int(rts[0]))
+    pw = s.power_history(int(rts[0]))
+    assert abs(pw[-1] - pw[-51]) <= 1e-3 * pw[-1]
+    if len(pw) > 100:
+        assert abs(pw[-51] - pw[-101]) > 1e-3 * pw[-51]  # not converged at the previous check
+    for a, c in zip(b.download(0), s.download()):
+        if a is not None:
+            assert rel_l2(a, c) < 1e-14
+
+
+def test_reference_instability_is_reproduced_and_detected(gpu):
+    """Two equations on a coarse crystal grid (8 slices, 31x threshold) make the
+    reference scheme itself blow up; the oracle and the HIP solver go
+    non-finite at the same round trip, and the batched run reports the
+    instance as diverged (opo_batch_run -> OPO_ERR_DIVERGED, rts = -2)."""
+    P = gpu
+    cfg = c1(N=1024, NZ=8, NRT=1100)
+    p, T, w, Ap_in, Ap, As, Ai = inputs(cfg, seed=5, sigma=1e-20)
+    _, opw = O.run(p, w, T, Ap_in, Ap.copy(), As.copy(), None, 0, 1100, power=True)
+    o_bad = int(np.argmax(~np.isfinite(opw)))
+    assert not np.isfinite(opw[o_bad])
+    s = P.Solver(P.params_from(P.derive(to_args(cfg))))
+    s.upload(w, T, Ap_in, Ap, As, None)
+    s.run(0, 1100)
+    gpw = s.power_history(1100)
+    g_bad = int(np.argmax(~np.isfinite(gpw)))
+    assert g_bad == o_bad
+    assert np.allclose(gpw[:o_bad - 30], opw[:o_bad - 30], rtol=1e-6)
+    pb, xb = instance(P, cfg, 5, 1e-20)
+    b = P.Batch([pb])
+    b.upload(0, xb["w"], xb["T"], xb["Ap_in"], xb["Ap"], xb["As"], xb["Ai"])
+    with pytest.raises(P.OpoError) as e:
+        b.run(1100, check_every=50, tol=1e-9)
+    assert e.value.status == P.ERR_DIVERGED
+
+
+@pytest.mark.parametrize("which", ["c1", "c2", "c5"])
+def test_fp32c_error_against_fp64_oracle(gpu, which):
+    """FP32-complex fields/FFT/RK with FP64 QPM phase, detuning phase and power
+    reduction: the relative L2 error against the FP64 oracle is recorded per
+    configuration (gpurun_out/fp32c_errors.json) and bounded."""
+    P = gpu
+    make = {"c1": lambda: c1(N=4096, NZ=32, NRT=200), "c2": lambda: c2(N=16384, NZ=20, NRT=200),
+            "c5": lambda: c5(N=4096, NZ=16, NRT=40)}[which]
+    cfg = make()
+    nrt = 20
+    p, T, w, Ap_in, Ap, As, Ai = inputs(cfg, seed=4, sigma=1e-2)
+    s32 = P.Solver(P.params_from(P.derive(to_args(cfg)), P.FP32C))
+    s32.upload(w, T, Ap_in, Ap, As, Ai)
+    s32.run(0, 1)
+    g1 = s32.download()
+    A1 = [Ap.copy(), As.copy(), None if Ai is None else Ai.copy()]
+    O.run(p, w, T, Ap_in, A1[0], A1[1], A1[2], 0, 1)
+    s32.set_fields(Ap, As, Ai)
+    s32.run(0, nrt)
+    gN = s32.download()
+    AN = [Ap.copy(), As.copy(), None if Ai is None else Ai.copy()]
+    O.run(p, w, T, Ap_in, AN[0], AN[1], AN[2], 0, nrt)
+    err1 = {f: rel_l2(g1[i], A1[i]) for i, f in enumerate("psi") if A1[i] is not None}
+    errN = {f: rel_l2(gN[i], AN[i]) for i, f in enumerate("psi") if AN[i] is not None}
+    print(which, "FP32C rel L2 after 1 RT", err1, f"after {nrt} RT", errN)
+    os.makedirs("gpurun_out", exist_ok=True)
+    path = "gpurun_out/fp32c_errors.json"
+    rec = json.load(open(path)) if os.path.exists(path) else {}
+    rec[which] = {"after_1_rt": err1, f"after_{nrt}_rt": errN}
+    json.dump(rec, open(path, "w"), indent=1)
+    assert max(err1.values()) < 1e-5
+    assert max(errN.values()) < 1e-3
+
+
+def test_nonfinite_power_marks_instance_diverged(gpu):
+    P = gpu
+    cfg = c2(N=1024, NZ=4, NRT=100)
+    p, x = instance(P, cfg, 1, 1e-3)
+    bad = x["As"].copy()
+    bad[17] = np.nan
+    p2, x2 = instance(P, cfg, 2, 1e-3)
+    b = P.Batch([p, p2])
+    b.upload(0, x["w"], x["T"], x["Ap_in"], x["Ap"], bad, x["Ai"])
+    b.upload(1, x2["w"], x2["T"], x2["Ap_in"], x2["Ap"], x2["As"], x2["Ai"])
+    with pytest.raises(P.OpoError) as e:
+        b.run(60, check_every=50, tol=1e-12)
+    assert e.value.status == P.ERR_DIVERGED
+
+
+def test_invalid_calls_fail_cleanly(gpu):
+    P = gpu
+    cfg = c1(N=1024, NZ=4, NRT=5)
+    s = P.Solver(P.params_from(P.derive(to_args(cfg))))
+    with pytest.raises(P.OpoError) as e:
+        s.run(0, 2)  # before upload
+    assert e.value.status == P.ERR_STATE
+    p, T, w, Ap_in, Ap, As, Ai = inputs(cfg)
+    with pytest.raises(P.OpoError):
+        s.upload(w, T, Ap_in[:10], Ap, As, Ai)  # pump too short
+    with pytest.raises(P.OpoError):
+        s.nl_half(99, 1)
+    with pytest.raises(P.OpoError):
+        s.set_tiles(48, 0)
