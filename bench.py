@@ -68,6 +68,61 @@ def cpu_baseline(args_c2, seconds=12.0):
                       "(radix-2 FFT, no FFTW on the host), single thread"}
 
 
+def sweep_section(weights):
+    """Sweep points/s on the 256-point continuous-wave three-equation sweep
+    family (cuopo_amd.sweep): batched FP32-complex from noise, batched from the
+    surrogate's warm start, and the plain per-instance FP64 solver."""
+    import numpy as np
+    import cuopo_amd as P
+    from cuopo_amd import sweep as SW
+    pts = SW.sweep_points(256, seed=3)
+    out = {"points": len(pts), "family": "CW three-equation MgO:PPLN, 20 mm, 100 slices/pass, N in 2^12..2^15, "
+                                          "pump 1.2-5x threshold, detuning U[-pi,pi] rad, R U[0.90,0.99]; "
+                                          "finished = steady (power change <1e-6 over 50 round trips) or off "
+                                          "(decayed below threshold) or 5000-round-trip cap"}
+
+    def summarise(r):
+        st = r["states"]
+        rts = np.where(r["rts"] > 0, r["rts"], 5000)
+        return {"run_s": r["run_s"], "points_per_s": len(pts) / r["run_s"], "steady": int((st == 1).sum()),
+                "off": int((st == 2).sum()), "capped": int((st == 0).sum()), "diverged": int((st == -1).sum()),
+                "mean_round_trips": float(rts.mean())}, rts
+
+    cold = SW.run_batched(pts, precision=P.FP32C)
+    out["batched_fp32c_cold"], cold_rts = summarise(cold)
+    cold_power, cold_states, ap0 = cold["power"], cold["states"], cold["ap0"]
+    cold["batch"].close()
+    # plain FP64 per-instance solver: measured round-trip time per grid size
+    per_rt = {}
+    for N in SW.SWEEP_N:
+        pt = next(p for p in pts if p[0] == N)
+        o = SW.run_single_fp64(pt, max_rt=200, tol=0.0)
+        per_rt[N] = o["wall_s"] / o["rts"]
+    est = float(sum(cold_rts[k] * per_rt[p[0]] for k, p in enumerate(pts)))
+    out["per_instance_fp64_cold"] = {"seconds_per_round_trip": {str(k): v for k, v in per_rt.items()},
+                                     "estimated_sweep_s": est, "points_per_s": len(pts) / est,
+                                     "estimate": "sum over points of the round trips each point ran in the "
+                                                 "batched cold run x the FP64 per-instance round-trip time "
+                                                 "measured at its grid size (200 round trips per size)"}
+    out["speedup_batched_cold_vs_per_instance"] = out["batched_fp32c_cold"]["points_per_s"] / (len(pts) / est)
+    if weights and os.path.exists(weights):
+        from cuopo_amd.train_surrogate import load
+        sur, meta = load(weights)
+        warm = sur.warm_fields_mixed([p[1:] for p in pts], [p[0] for p in pts], ap0, seed=11)
+        w = SW.run_batched(pts, precision=P.FP32C, warm=warm)
+        out["batched_fp32c_warm"], warm_rts = summarise(w)
+        both = (cold_states == 1) & (w["states"] == 1)
+        rel = np.abs(w["power"] - cold_power) / np.maximum(cold_power, 1e-300)
+        out["batched_fp32c_warm"]["rel_power_vs_cold_where_both_steady"] = \
+            {"count": int(both.sum()), "median": float(np.median(rel[both])) if both.any() else None,
+             "max": float(rel[both].max()) if both.any() else None}
+        out["batched_fp32c_warm"]["same_finish_state_as_cold"] = int((w["states"] == cold_states).sum())
+        out["surrogate"] = meta
+        out["speedup_batched_warm_vs_per_instance"] = out["batched_fp32c_warm"]["points_per_s"] / (len(pts) / est)
+        w["batch"].close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +130,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "cuopo_amd", "data", "surrogate.npz"))
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this solver runs one instance per GPU and is benchmarked on one GPU; "
@@ -136,6 +193,8 @@ def main():
                   "copy_probe_GBps": copy_gbps,
                   "intracavity_power_end": float(np.sum(np.abs(As) ** 2))},
     }
+    if not a.no_sweep:
+        out["extra"]["sweep"] = sweep_section(a.weights)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, a.cpu_seconds)
     print(json.dumps(out))

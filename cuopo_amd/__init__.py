@@ -116,11 +116,21 @@ SIGNATURES = {
     "opo_batch_create": (_i, [C.POINTER(Params), _i, C.POINTER(_p)]),
     "opo_batch_upload": (_i, [_p, _i, _p, _p, _p, _sz, _p, _p, _p]),
     "opo_batch_run": (_i, [_p, _i, _i, _d, _p]),
+    "opo_batch_run_ex": (_i, [_p, _i, _i, _d, _d, _p, _p]),
     "opo_batch_download": (_i, [_p, _i, _p, _p, _p]),
     "opo_batch_power": (_i, [_p, _p]),
+    "opo_batch_power_history": (_i, [_p, _i, _p, _i]),
     "opo_batch_last_run_ms": (_i, [_p, C.POINTER(_d)]),
     "opo_batch_set_tiles": (_i, [_p, _i, _i]),
     "opo_batch_destroy": (_i, [_p]),
+    "opo_mlp_create": (_i, [_i, _i, _i, _i, _i, _u64, _i, C.POINTER(_p)]),
+    "opo_mlp_param_count": (_i, [_p, C.POINTER(_sz)]),
+    "opo_mlp_set_weights": (_i, [_p, _p, _sz]),
+    "opo_mlp_get_weights": (_i, [_p, _p, _sz]),
+    "opo_mlp_forward": (_i, [_p, _p, _i, _p]),
+    "opo_mlp_gradients": (_i, [_p, _p, _p, _i, _p, _p]),
+    "opo_mlp_train_step": (_i, [_p, _p, _p, _i, C.c_float, _p]),
+    "opo_mlp_destroy": (_i, [_p]),
     "opo_strerror": (C.c_char_p, [_i]),
     "opo_last_error": (C.c_char_p, []),
     "opo_device_count": (_i, [C.POINTER(_i)]),
@@ -383,6 +393,16 @@ class Batch:
         _check(lib().opo_batch_run(self._h, max_rt, check_every, tol, _ptr(rts)))
         return rts
 
+    def run_ex(self, max_rt, check_every=50, tol=1e-6, off_ratio=1e-3):
+        """Returns (round trips run, state) per instance; state 1 steady,
+        2 off (decayed below threshold), 0 not converged, -1 diverged."""
+        rts = np.zeros(self.count, np.int32)
+        st = np.zeros(self.count, np.int32)
+        rc = lib().opo_batch_run_ex(self._h, max_rt, check_every, tol, off_ratio, _ptr(rts), _ptr(st))
+        if rc not in (OK, ERR_DIVERGED):
+            _check(rc)
+        return rts, st
+
     def download(self, idx):
         N = self.params[idx].N
         three = bool(self.params[idx].three_eqs)
@@ -394,6 +414,11 @@ class Batch:
     def power(self):
         out = np.empty(self.count)
         _check(lib().opo_batch_power(self._h, _ptr(out)))
+        return out
+
+    def power_history(self, idx, n):
+        out = np.zeros(n)
+        _check(lib().opo_batch_power_history(self._h, idx, _ptr(out), n))
         return out
 
     def set_tiles(self, col_threads=0, row_threads=0):

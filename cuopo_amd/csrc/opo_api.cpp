@@ -87,7 +87,7 @@ struct Group {
   void *tw1 = nullptr, *tw2 = nullptr, *twlo = nullptr, *twhi = nullptr;
   OpoInst *d_insts = nullptr;
   int *d_rt = nullptr, *d_active = nullptr, *d_conv = nullptr, *d_status = nullptr;
-  double *d_partials = nullptr, *d_pref = nullptr, *d_hist = nullptr;
+  double *d_partials = nullptr, *d_pref = nullptr, *d_pfirst = nullptr, *d_rprev = nullptr, *d_hist = nullptr;
   int hist_len = 0;
   int ct = 0, rt = 0;  // column tile (columns per block), row tile (rows per block)
   bool any_gdd = false;
@@ -107,6 +107,7 @@ struct Engine {
   float last_ms = 0;
   int last_rt0 = 0, last_nrt = 0;
   int col_threads = 0, row_threads = 0;
+  double off_ratio = 1e-6;
   bool inst_dirty = true;
   std::vector<std::pair<int, int>> where;  // batch index -> (group, slot)
 
@@ -238,6 +239,8 @@ int make_group(Engine &E, Group &g) {
   RCHK(dalloc(g, (void **)&g.d_conv, n * sizeof(int)));
   RCHK(dalloc(g, (void **)&g.d_status, n * sizeof(int)));
   RCHK(dalloc(g, (void **)&g.d_pref, n * sizeof(double)));
+  RCHK(dalloc(g, (void **)&g.d_pfirst, n * sizeof(double)));
+  RCHK(dalloc(g, (void **)&g.d_rprev, n * sizeof(double)));
   RCHK(dalloc(g, (void **)&g.d_partials, (size_t)n * g.N2 * sizeof(double)));
   g.hist_len = std::max(E.NRT, 1);
   RCHK(dalloc(g, (void **)&g.d_hist, (size_t)n * g.hist_len * sizeof(double)));
@@ -400,7 +403,7 @@ int sync_insts(Engine &E) {
       d.mir[0] = std::cos(kPi * p.delta); d.mir[1] = std::sin(kPi * p.delta);
       d.rt = g.d_rt + k; d.active = g.d_active + k; d.partials = g.d_partials + (size_t)k * g.N2;
       d.power_hist = g.d_hist + (size_t)k * g.hist_len; d.hist_len = g.hist_len;
-      d.conv_rt = g.d_conv + k; d.p_ref = g.d_pref + k; d.status = g.d_status + k;
+      d.conv_rt = g.d_conv + k; d.p_ref = g.d_pref + k; d.p_first = g.d_pfirst + k; d.r_prev = g.d_rprev + k; d.status = g.d_status + k;
       d.snap = I.snap;
     }
     HIPCHK(hipMemcpyAsync(g.d_insts, h.data(), n * sizeof(OpoInst), hipMemcpyHostToDevice, E.stream));
@@ -443,6 +446,7 @@ int emit_row(Engine &E, Group &g, bool gdd, int mask, hipEvent_t e0 = nullptr, h
 int emit_tick(Engine &E, Group &g, int check_every, double tol) {
   opo::TickArgs a{};
   a.count = (int)g.members.size(); a.tiles = g.N2 / g.ct; a.check_every = check_every; a.tol = tol;
+  a.off_ratio = E.off_ratio;
   int e = opo::launch_tick(g.d_insts, a, E.stream);
   if (e != 0) return fail(OPO_ERR_HIP, std::string("tick launch: ") + hipGetErrorString((hipError_t)e));
   return OPO_OK;
@@ -488,6 +492,8 @@ int reset_counters(Engine &E, int rt0) {
     HIPCHK(hipMemcpyAsync(g.d_conv, conv.data(), n * sizeof(int), hipMemcpyHostToDevice, E.stream));
     HIPCHK(hipMemcpyAsync(g.d_status, stat.data(), n * sizeof(int), hipMemcpyHostToDevice, E.stream));
     HIPCHK(hipMemcpyAsync(g.d_pref, pref.data(), n * sizeof(double), hipMemcpyHostToDevice, E.stream));
+    HIPCHK(hipMemcpyAsync(g.d_pfirst, pref.data(), n * sizeof(double), hipMemcpyHostToDevice, E.stream));
+    HIPCHK(hipMemcpyAsync(g.d_rprev, pref.data(), n * sizeof(double), hipMemcpyHostToDevice, E.stream));
   }
   HIPCHK(hipStreamSynchronize(E.stream));
   return OPO_OK;
@@ -608,6 +614,10 @@ int engine_init(Engine &E, const opo_params *params, int count) {
 }
 
 }  // namespace
+
+namespace opo {
+void set_error(const std::string &msg) { g_err = msg; }
+}  // namespace opo
 
 // ================================================================== C ABI
 
@@ -943,12 +953,15 @@ int opo_batch_set_tiles(opo_batch *b, int col_threads, int row_threads) {
   return OPO_OK;
 }
 
-int opo_batch_run(opo_batch *b, int max_rt, int check_every, double tol, int *rts_out) {
+int opo_batch_run_ex(opo_batch *b, int max_rt, int check_every, double tol, double off_ratio, int *rts_out,
+                     int *states_out) {
   if (!b) return fail(OPO_ERR_INVALID, "null batch");
   if (max_rt < 1) return fail(OPO_ERR_INVALID, "max_rt must be >= 1");
   Engine &E = b->E;
   HIPCHK(hipSetDevice(E.device));
   if (tol <= 0) check_every = 0;
+  E.off_ratio = off_ratio;
+  E.graph_key = -1;
   RCHK(engine_run(E, 0, max_rt, check_every, tol));
   int rc = OPO_OK;
   for (auto &g : E.groups) {
@@ -957,12 +970,17 @@ int opo_batch_run(opo_batch *b, int max_rt, int check_every, double tol, int *rt
     HIPCHK(hipMemcpy(conv.data(), g.d_conv, n * sizeof(int), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(stat.data(), g.d_status, n * sizeof(int), hipMemcpyDeviceToHost));
     for (int k = 0; k < n; ++k) {
-      if (rts_out) rts_out[g.members[k]] = stat[k] != 0 ? -2 : conv[k];
-      if (stat[k] != 0) rc = OPO_ERR_DIVERGED;
+      if (rts_out) rts_out[g.members[k]] = stat[k] < 0 ? -2 : conv[k];
+      if (stat[k] < 0) rc = OPO_ERR_DIVERGED;
+      if (states_out) states_out[g.members[k]] = stat[k] < 0 ? -1 : (conv[k] < 0 ? 0 : (stat[k] == 1 ? 2 : 1));
     }
   }
   if (rc != OPO_OK) return fail(rc, "non-finite intracavity power in at least one instance");
   return OPO_OK;
+}
+
+int opo_batch_run(opo_batch *b, int max_rt, int check_every, double tol, int *rts_out) {
+  return opo_batch_run_ex(b, max_rt, check_every, tol, 1e-6, rts_out, nullptr);
 }
 
 int opo_batch_download(opo_batch *b, int idx, double *Ap, double *As, double *Ai) {
@@ -978,6 +996,16 @@ int opo_batch_download(opo_batch *b, int idx, double *Ap, double *As, double *Ai
   if (Ap) std::memcpy(Ap, st.data(), g.N * sizeof(cd));
   if (As) std::memcpy(As, st.data() + g.N, g.N * sizeof(cd));
   if (Ai && g.NF == 3) std::memcpy(Ai, st.data() + 2 * (size_t)g.N, g.N * sizeof(cd));
+  return OPO_OK;
+}
+
+int opo_batch_power_history(opo_batch *b, int idx, double *out, int n) {
+  if (!b || !out || idx < 0 || idx >= b->E.count || n < 0) return fail(OPO_ERR_INVALID, "bad arguments");
+  Engine &E = b->E;
+  auto [gi, k] = E.where[idx];
+  Group &g = E.groups[gi];
+  n = std::min(n, g.hist_len);
+  HIPCHK(hipMemcpy(out, g.d_hist + (size_t)k * g.hist_len, n * sizeof(double), hipMemcpyDeviceToHost));
   return OPO_OK;
 }
 

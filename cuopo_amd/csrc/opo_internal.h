@@ -52,6 +52,8 @@ struct OpoInst {
   int hist_len;
   int *conv_rt;            // round trip at which convergence was declared (-1)
   double *p_ref;           // power at the previous convergence check
+  double *p_first;         // power at the first convergence check (off-state test)
+  double *r_prev;          // P(check k-1) / P(check k-2)
   int *status;             // 0 ok, -5 non-finite power
   void *snap;              // NF*N complex: time-domain fields at the last check
   int pad_;
@@ -82,6 +84,7 @@ struct TickArgs {
   int tiles;         // column tiles per instance (partials per instance)
   int check_every;   // 0 = no convergence test
   double tol;
+  double off_ratio;  // "off" test: P < off_ratio * P(first check), see tick_kernel
 };
 
 // Launchers (defined in opo_kernels.hip).  Return hipError_t as int.

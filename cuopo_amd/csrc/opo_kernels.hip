@@ -342,9 +342,22 @@ __global__ void tick_kernel(const OpoInst *__restrict__ insts, TickArgs a) {
   if (a.check_every > 0 && done % a.check_every == 0) {
     const double prev = *I->p_ref;
     if (prev > 0 && P > 0 && fabs(P - prev) <= a.tol * P) {
-      *I->conv_rt = done;
+      *I->conv_rt = done;  // steady state
       *I->active = 0;
+    } else if (prev > 0 && *I->p_first > 0) {
+      // Below threshold: in the small-signal regime P is a sum of decaying or
+      // growing exponentials, so log P is convex and the window ratio r never
+      // decreases; once r < 1 has stopped changing the slowest mode dominates
+      // and decays.  Require also a collapse below off_ratio x P(first check).
+      const double r = P / prev, rp = *I->r_prev;
+      if (r < 1.0 && rp > 0 && fabs(r - rp) <= 1e-3 * r && P < a.off_ratio * *I->p_first) {
+        *I->conv_rt = done;
+        *I->status = 1;
+        *I->active = 0;
+      }
     }
+    if (prev > 0) *I->r_prev = P / prev;
+    if (*I->p_first == 0) *I->p_first = P;
     *I->p_ref = P;
   }
 }

@@ -1,0 +1,104 @@
+"""Batched parameter sweeps: the work src/cuOPO.sh:91-175 runs as one process
+per point, run here as one batch of instances on one GPU.
+
+The sweep family (continuous wave, three equations, MgO:PPLN, 532 nm pump,
+20 mm crystal, 100 z-slices per pass) draws per point: grid size N from
+{2^12, 2^13, 2^14, 2^15}, pump power uniform in [1.2, 5] x threshold, cavity
+detuning uniform in [-pi, pi] rad and signal reflectivity uniform in
+[0.90, 0.99].  A point is finished when its intracavity signal power changes
+by less than `tol` over `check_every` round trips (steady state), when it has
+decayed below threshold (off: power a factor 1/off_ratio below its first
+check with a settled decay rate; seeded white noise can lose up to ~N before
+a growing mode dominates, so off_ratio = 1e-6 leaves margin for N <= 2^15),
+or at `max_rt`.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+from . import FP32C, FP64, Batch, Solver, derive, grids, noise, params_from, pump_input
+from .surrogate import sweep_args
+
+SWEEP_N = (1 << 12, 1 << 13, 1 << 14, 1 << 15)
+
+
+def sweep_points(count=256, seed=3):
+    """Rows of (N, nth_rel, delta_rad, lcr_mm, gdd_fs2, R)."""
+    rng = np.random.default_rng(seed)
+    N = rng.choice(SWEEP_N, count)
+    nth = rng.uniform(1.2, 5.0, count)
+    delta = rng.uniform(-math.pi, math.pi, count)
+    R = rng.uniform(0.90, 0.99, count)
+    return [(int(N[k]), float(nth[k]), float(delta[k]), 20.0, 0.0, float(R[k])) for k in range(count)]
+
+
+def _setup(pt, NZ, max_rt, precision, seed, k, sigma, warm):
+    N, nth, delta, lcr, gdd, R = pt
+    a = sweep_args(nth, delta, lcr, gdd, R, N=N, NZ=NZ, NRT=max_rt)
+    d = derive(a)
+    p = params_from(d, precision)
+    T, w = grids(d)
+    Ap_in = pump_input(d)
+    if warm is not None:
+        As, Ai = warm[k]
+    else:
+        As = noise(seed + k, 0, sigma, N)
+        Ai = noise(seed + k, 1, sigma, N)
+    return p, d, (w, T, Ap_in, Ap_in[:N].copy(), As, Ai)
+
+
+def run_batched(points, NZ=100, max_rt=5000, check_every=50, tol=1e-6, off_ratio=1e-6, precision=FP32C,
+                seed=1000, sigma=1e-20, warm=None, tiles=(0, 0)):
+    """All points in one batch.  Returns a dict with per-point round trips,
+    states (1 steady, 2 off, 0 capped, -1 diverged), final powers, the
+    device time of the run and the wall time including uploads."""
+    t0 = time.perf_counter()
+    made = [_setup(pt, NZ, max_rt, precision, seed, k, sigma, warm) for k, pt in enumerate(points)]
+    b = Batch([m[0] for m in made])
+    if tiles != (0, 0):
+        b.set_tiles(*tiles)
+    for k, m in enumerate(made):
+        b.upload(k, *m[2])
+    t1 = time.perf_counter()
+    rts, st = b.run_ex(max_rt, check_every=check_every, tol=tol, off_ratio=off_ratio)
+    t2 = time.perf_counter()
+    power = b.power()
+    out = dict(rts=rts, states=st, power=power, device_ms=b.last_run_ms(), run_s=t2 - t1, total_s=t2 - t0,
+               ap0=np.array([m[1].Ap0 for m in made]), batch=b)
+    return out
+
+
+def run_single_fp64(pt, NZ=100, max_rt=5000, check_every=50, tol=1e-6, off_ratio=1e-6, seed=1000, k=0,
+                    sigma=1e-20):
+    """Plain per-instance FP64 cold start: one solver, round trips in chunks of
+    check_every with the same convergence and off tests evaluated on the host
+    from the power history (the way a per-point driver would)."""
+    p, d, up = _setup(pt, NZ, max_rt, FP64, seed, k, sigma, None)
+    s = Solver(p)
+    s.upload(*up)
+    t0 = time.perf_counter()
+    done, state, prev, first, rprev = 0, 0, 0.0, 0.0, 0.0
+    while done < max_rt:
+        n = min(check_every, max_rt - done)
+        s.run(done, n)  # the fields stay on the device between calls
+        done += n
+        P = float(s.power_history(n)[-1])
+        if prev > 0 and P > 0 and abs(P - prev) <= tol * P:
+            state = 1
+            break
+        if prev > 0 and first > 0:
+            r = P / prev
+            if r < 1 and rprev > 0 and abs(r - rprev) <= 1e-3 * r and P < off_ratio * first:
+                state = 2
+                break
+        if prev > 0:
+            rprev = P / prev
+        if first == 0:
+            first = P
+        prev = P
+    wall = time.perf_counter() - t0
+    s.close()
+    return dict(rts=done, state=state, power=P, wall_s=wall)

@@ -169,12 +169,38 @@ int opo_batch_upload(opo_batch *b, int idx, const double *w, const double *T, co
  * rts_out[k] receives the round trips instance k ran (its convergence point),
  * or -1 if it did not converge.  tol <= 0 runs exactly max_rt round trips. */
 int opo_batch_run(opo_batch *b, int max_rt, int check_every, double tol, int *rts_out);
+/* (opo_batch_run applies the off test below with off_ratio = 1e-6.)
+ * As opo_batch_run, and an instance whose power has fallen below
+ * off_ratio x its power at the first check (and is still falling) is stopped
+ * as "off" (below threshold).  states_out[k]: 1 steady state, 2 off, 0 not
+ * converged, -1 diverged; rts_out[k] the round trips it ran (-1 / -2 as above). */
+int opo_batch_run_ex(opo_batch *b, int max_rt, int check_every, double tol, double off_ratio, int *rts_out,
+                     int *states_out);
 int opo_batch_download(opo_batch *b, int idx, double *Ap, double *As, double *Ai);
+/* sum |As|^2 after each round trip of instance idx's last run (n values) */
+int opo_batch_power_history(opo_batch *b, int idx, double *out, int n);
 /* final sum |As|^2 per instance (count values) */
 int opo_batch_power(opo_batch *b, double *out);
 int opo_batch_last_run_ms(opo_batch *b, double *ms);
 int opo_batch_set_tiles(opo_batch *b, int col_threads, int row_threads);
 int opo_batch_destroy(opo_batch *b);
+
+/* ---------------- warm-start surrogate (no reference counterpart) ----------
+ * MLP n_in -> width (GELU) x depth -> n_out with BF16 weights/activations and
+ * FP32 accumulation on MFMA; FP32 master weights and Adam.  The flat parameter
+ * vector holds, layer by layer, W [dout x din] row-major then b [dout].
+ * Inputs/outputs are row-major FP32 host arrays [batch x n]. */
+typedef struct opo_mlp opo_mlp;
+int opo_mlp_create(int n_in, int width, int depth, int n_out, int max_batch, uint64_t seed, int device,
+                   opo_mlp **out);
+int opo_mlp_param_count(opo_mlp *m, size_t *n);
+int opo_mlp_set_weights(opo_mlp *m, const float *w, size_t n);
+int opo_mlp_get_weights(opo_mlp *m, float *w, size_t n);
+int opo_mlp_forward(opo_mlp *m, const float *x, int batch, float *y);
+/* gradient of the mean-squared error loss (grad: param_count floats, may be NULL) */
+int opo_mlp_gradients(opo_mlp *m, const float *x, const float *target, int batch, float *grad, float *loss);
+int opo_mlp_train_step(opo_mlp *m, const float *x, const float *target, int batch, float lr, float *loss);
+int opo_mlp_destroy(opo_mlp *m);
 
 /* ---------------- errors ---------------- */
 /* Streaming copy of `bytes` between two device buffers, iters times:

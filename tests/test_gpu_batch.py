@@ -74,7 +74,7 @@ def test_convergence_snapshot_is_the_converged_state(gpu):
     """An instance declared converged at round trip r (relative power change
     over check_every round trips below tol) is frozen; its downloaded fields
     equal a single-instance run of exactly r round trips.  A below-threshold
-    instance in the same batch never converges."""
+    instance in the same batch stops as "off" once its decay rate is settled."""
     P = gpu
     cfg = c2(N=1024, NZ=8, NRT=3000)
     cfg.Nth = 10.0
@@ -85,9 +85,11 @@ def test_convergence_snapshot_is_the_converged_state(gpu):
     b = P.Batch([p, p2])
     b.upload(0, x["w"], x["T"], x["Ap_in"], x["Ap"], x["As"], x["Ai"])
     b.upload(1, x2["w"], x2["T"], x2["Ap_in"], x2["Ap"], x2["As"], x2["Ai"])
-    rts = b.run(3000, check_every=50, tol=1e-3)
-    assert rts[0] > 0 and rts[0] % 50 == 0
-    assert rts[1] == -1
+    rts, st = b.run_ex(3000, check_every=50, tol=1e-3, off_ratio=1e-6)
+    assert rts[0] > 0 and rts[0] % 50 == 0 and st[0] == 1
+    assert st[1] == 2 and 0 < rts[1] < 3000  # the below-threshold instance stops as "off"
+    pw1 = b.power_history(1, int(rts[1]))
+    assert pw1[-1] < 1e-6 * pw1[49] and pw1[-1] < pw1[-51]
     s = P.Solver(p)
     s.upload(x["w"], x["T"], x["Ap_in"], x["Ap"], x["As"], x["Ai"])
     s.run(0, int(rts[0]))
