@@ -92,6 +92,9 @@ struct Group {
   int ct = 0, rt = 0;  // column tile (columns per block), row tile (rows per block)
   bool any_gdd = false;
   std::vector<void *> allocs;
+  hipStream_t gs = nullptr;      // the group's own stream (batches with several grid sizes)
+  hipEvent_t done_ev = nullptr;  // join point of the group's chain
+  hipStream_t cur = nullptr;     // stream the stage emitters launch on
 };
 
 struct Engine {
@@ -111,11 +114,17 @@ struct Engine {
   bool inst_dirty = true;
   std::vector<std::pair<int, int>> where;  // batch index -> (group, slot)
 
+  hipEvent_t fork_ev = nullptr;
+
   ~Engine() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
-    for (auto &g : groups)
+    for (auto &g : groups) {
       for (void *p : g.allocs) (void)hipFree(p);
+      if (g.done_ev) (void)hipEventDestroy(g.done_ev);
+      if (g.gs) (void)hipStreamDestroy(g.gs);
+    }
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -405,6 +414,7 @@ int sync_insts(Engine &E) {
       d.power_hist = g.d_hist + (size_t)k * g.hist_len; d.hist_len = g.hist_len;
       d.conv_rt = g.d_conv + k; d.p_ref = g.d_pref + k; d.p_first = g.d_pfirst + k; d.r_prev = g.d_rprev + k; d.status = g.d_status + k;
       d.snap = I.snap;
+      d.active_v = 1;
     }
     HIPCHK(hipMemcpyAsync(g.d_insts, h.data(), n * sizeof(OpoInst), hipMemcpyHostToDevice, E.stream));
   }
@@ -425,7 +435,7 @@ int emit_col(Engine &E, Group &g, unsigned ops, int load, int inv, int fwd, int 
   a.s_end = s_end; a.s_start = s_start; a.pump_off = pump_off; a.three_eqs = E.three_eqs;
   a.snap_every = snap_every;
   const int threads_ct = g.ct;
-  int e = opo::launch_col(g.fp32, g.N1, g.NF, threads_ct, (int)g.members.size(), g.d_insts, a, E.stream, e0, e1);
+  int e = opo::launch_col(g.fp32, g.N1, g.NF, threads_ct, (int)g.members.size(), g.d_insts, a, g.cur, e0, e1);
   if (e != 0) return fail(OPO_ERR_HIP, std::string("col_stage launch: ") + hipGetErrorString((hipError_t)e));
   return OPO_OK;
 }
@@ -438,7 +448,7 @@ int emit_row(Engine &E, Group &g, bool gdd, int mask, hipEvent_t e0 = nullptr, h
   for (int f = 0; f < g.NF; ++f)
     if ((mask >> f) & 1) fl[nf++] = f;
   a.nf = nf; a.f0 = fl[0]; a.f1 = fl[1]; a.f2 = fl[2];
-  int e = opo::launch_row(g.fp32, g.N2, g.NF, g.rt, (int)g.members.size(), 0, g.d_insts, a, E.stream, e0, e1);
+  int e = opo::launch_row(g.fp32, g.N2, g.NF, g.rt, (int)g.members.size(), 0, g.d_insts, a, g.cur, e0, e1);
   if (e != 0) return fail(OPO_ERR_HIP, std::string("row_stage launch: ") + hipGetErrorString((hipError_t)e));
   return OPO_OK;
 }
@@ -447,7 +457,7 @@ int emit_tick(Engine &E, Group &g, int check_every, double tol) {
   opo::TickArgs a{};
   a.count = (int)g.members.size(); a.tiles = g.N2 / g.ct; a.check_every = check_every; a.tol = tol;
   a.off_ratio = E.off_ratio;
-  int e = opo::launch_tick(g.d_insts, a, E.stream);
+  int e = opo::launch_tick(g.d_insts, a, g.cur);
   if (e != 0) return fail(OPO_ERR_HIP, std::string("tick launch: ") + hipGetErrorString((hipError_t)e));
   return OPO_OK;
 }
@@ -483,6 +493,8 @@ int emit_rt_body(Engine &E, Group &g, bool last, int check_every, double tol) {
 }
 
 int reset_counters(Engine &E, int rt0) {
+  E.inst_dirty = true;
+  RCHK(sync_insts(E));  // descriptors carry active_v = 1
   for (auto &g : E.groups) {
     const int n = (int)g.members.size();
     std::vector<int> rt(n, rt0), act(n, 1), conv(n, -1), stat(n, 0);
@@ -499,17 +511,44 @@ int reset_counters(Engine &E, int rt0) {
   return OPO_OK;
 }
 
+// Batches with several grid sizes run each group's chain on the group's own
+// stream (forked from and joined back to the engine stream), so the hardware
+// queues overlap the groups' kernels; one group uses the engine stream.
+int fork_groups(Engine &E) {
+  if (E.groups.size() == 1) {
+    E.groups[0].cur = E.stream;
+    return OPO_OK;
+  }
+  HIPCHK(hipEventRecord(E.fork_ev, E.stream));
+  for (auto &g : E.groups) {
+    HIPCHK(hipStreamWaitEvent(g.gs, E.fork_ev, 0));
+    g.cur = g.gs;
+  }
+  return OPO_OK;
+}
+
+int join_groups(Engine &E) {
+  if (E.groups.size() == 1) return OPO_OK;
+  for (auto &g : E.groups) {
+    HIPCHK(hipEventRecord(g.done_ev, g.gs));
+    HIPCHK(hipStreamWaitEvent(E.stream, g.done_ev, 0));
+    g.cur = E.stream;
+  }
+  return OPO_OK;
+}
+
 int build_graph(Engine &E, int check_every, double tol) {
   const int key = check_every * 1000003 + (int)(tol * 1e12) + (E.save_enabled ? 7 : 0);
   if (E.gexec && E.graph_key == key) return OPO_OK;
   if (E.gexec) { (void)hipGraphExecDestroy(E.gexec); E.gexec = nullptr; }
   if (E.graph) { (void)hipGraphDestroy(E.graph); E.graph = nullptr; }
   HIPCHK(hipStreamBeginCapture(E.stream, hipStreamCaptureModeThreadLocal));
-  int rc = OPO_OK;
+  int rc = fork_groups(E);
   for (auto &g : E.groups) {
-    rc = emit_rt_body(E, g, false, check_every, tol);
     if (rc != OPO_OK) break;
+    rc = emit_rt_body(E, g, false, check_every, tol);
   }
+  if (rc == OPO_OK) rc = join_groups(E);
   hipGraph_t gr = nullptr;
   hipError_t ee = hipStreamEndCapture(E.stream, &gr);
   if (rc != OPO_OK) { if (gr) (void)hipGraphDestroy(gr); return rc; }
@@ -548,20 +587,30 @@ int engine_run(Engine &E, int rt0, int n_rt, int check_every, double tol) {
   const bool graph = E.use_graph && n_rt > 2;
   if (graph) RCHK(build_graph(E, check_every, tol));
   HIPCHK(hipEventRecord(E.ev0, E.stream));
+  RCHK(fork_groups(E));
   for (auto &g : E.groups) RCHK(emit_start(E, g));
+  RCHK(join_groups(E));
   int done = 0;
   bool alive = true;
   while (done < n_rt - 1 && alive) {
     const int seg = check_every > 0 ? std::min(check_every, n_rt - 1 - done) : n_rt - 1 - done;
     for (int r = 0; r < seg; ++r) {
-      if (graph) HIPCHK(hipGraphLaunch(E.gexec, E.stream));
-      else for (auto &g : E.groups) RCHK(emit_rt_body(E, g, false, check_every, tol));
+      if (graph) {
+        HIPCHK(hipGraphLaunch(E.gexec, E.stream));
+      } else {
+        RCHK(fork_groups(E));
+        for (auto &g : E.groups) RCHK(emit_rt_body(E, g, false, check_every, tol));
+        RCHK(join_groups(E));
+      }
     }
     done += seg;
     if (check_every > 0) RCHK(any_active(E, &alive));
   }
-  if (alive)
+  if (alive) {
+    RCHK(fork_groups(E));
     for (auto &g : E.groups) RCHK(emit_rt_body(E, g, true, check_every, tol));
+    RCHK(join_groups(E));
+  }
   HIPCHK(hipEventRecord(E.ev1, E.stream));
   HIPCHK(hipEventSynchronize(E.ev1));
   HIPCHK(hipEventElapsedTime(&E.last_ms, E.ev0, E.ev1));
@@ -605,8 +654,14 @@ int engine_init(Engine &E, const opo_params *params, int count) {
     E.where[k] = {gi, (int)E.groups[gi].members.size()};
     E.groups[gi].members.push_back(k);
   }
+  HIPCHK(hipEventCreateWithFlags(&E.fork_ev, hipEventDisableTiming));
   for (auto &g : E.groups) {
     for (int k : g.members) if (params[k].GDD != 0) g.any_gdd = true;
+    g.cur = E.stream;
+    if (E.groups.size() > 1) {
+      HIPCHK(hipStreamCreateWithFlags(&g.gs, hipStreamNonBlocking));
+      HIPCHK(hipEventCreateWithFlags(&g.done_ev, hipEventDisableTiming));
+    }
     RCHK(make_group(E, g));
     for (size_t j = 0; j < g.members.size(); ++j) g.inst[j].prm = params[g.members[j]];
   }

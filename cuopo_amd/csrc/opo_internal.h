@@ -56,6 +56,7 @@ struct OpoInst {
   double *r_prev;          // P(check k-1) / P(check k-2)
   int *status;             // 0 ok, -5 non-finite power
   void *snap;              // NF*N complex: time-domain fields at the last check
+  int active_v;            // copy of *active read by the stage kernels with the other fields
   int pad_;
 };
 
@@ -96,6 +97,6 @@ int launch_row(int fp32, int N2, int NF, int rt, int count, int tiles_y, const O
                const RowArgs &a, void *stream, void *ev0 = nullptr, void *ev1 = nullptr);
 // streaming copy of n 16-byte elements (bandwidth probe / counter calibration)
 int launch_copy(const void *src, void *dst, long long n, void *stream, void *ev0, void *ev1);
-int launch_tick(const OpoInst *insts, const TickArgs &a, void *stream);
+int launch_tick(OpoInst *insts, const TickArgs &a, void *stream);
 
 }  // namespace opo

@@ -100,7 +100,7 @@ __global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ ins
   V *lds = ltw + N1;                         // 2 buffers x NF fields x CT*N1
 
   const OpoInst *I = insts + blockIdx.y;
-  if (*I->active == 0) return;  // block-uniform
+  if (I->active_v == 0) return;  // block-uniform
 
   const int CT = 1 << a.log2ct;
   const int FT = CT * T1;
@@ -286,7 +286,7 @@ __global__ __launch_bounds__(512) void row_stage(const OpoInst *__restrict__ ins
   V *lds = ltw + N2;                         // 2 buffers x RT*N2
 
   const OpoInst *I = insts + blockIdx.y;
-  if (*I->active == 0) return;
+  if (I->active_v == 0) return;
 
   const int RT = 1 << a.log2rt;
   const int tiles_per_field = a.N1 >> a.log2rt;
@@ -323,11 +323,11 @@ __global__ __launch_bounds__(512) void row_stage(const OpoInst *__restrict__ ins
 // One thread per instance: reduce the column-tile partials in order, record
 // the power, advance the round-trip counter and, every check_every round
 // trips, test |P - P_prev| <= tol * P.
-__global__ void tick_kernel(const OpoInst *__restrict__ insts, TickArgs a) {
+__global__ void tick_kernel(OpoInst *__restrict__ insts, TickArgs a) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= a.count) return;
-  const OpoInst *I = insts + b;
-  if (*I->active == 0) return;
+  OpoInst *I = insts + b;
+  if (I->active_v == 0) return;
   double P = 0;
   for (int j = 0; j < a.tiles; ++j) P += I->partials[j];
   const int rt = *I->rt;
@@ -337,6 +337,7 @@ __global__ void tick_kernel(const OpoInst *__restrict__ insts, TickArgs a) {
   if (!(P == P) || P > 1.7e308) {  // NaN or Inf
     *I->status = -5;
     *I->active = 0;
+    I->active_v = 0;
     return;
   }
   if (a.check_every > 0 && done % a.check_every == 0) {
@@ -344,6 +345,7 @@ __global__ void tick_kernel(const OpoInst *__restrict__ insts, TickArgs a) {
     if (prev > 0 && P > 0 && fabs(P - prev) <= a.tol * P) {
       *I->conv_rt = done;  // steady state
       *I->active = 0;
+      I->active_v = 0;
     } else if (prev > 0 && *I->p_first > 0) {
       // Below threshold: in the small-signal regime P is a sum of decaying or
       // growing exponentials, so log P is convex and the window ratio r never
@@ -354,6 +356,7 @@ __global__ void tick_kernel(const OpoInst *__restrict__ insts, TickArgs a) {
         *I->conv_rt = done;
         *I->status = 1;
         *I->active = 0;
+        I->active_v = 0;
       }
     }
     if (prev > 0) *I->r_prev = P / prev;
@@ -467,7 +470,7 @@ int launch_copy(const void *src, void *dst, long long n, void *stream, void *ev0
   return (int)hipGetLastError();
 }
 
-int launch_tick(const OpoInst *insts, const TickArgs &a, void *stream) {
+int launch_tick(OpoInst *insts, const TickArgs &a, void *stream) {
   dim3 grid((a.count + 63) / 64);
   hipLaunchKernelGGL(tick_kernel, grid, dim3(64), 0, (hipStream_t)stream, insts, a);
   return (int)hipGetLastError();
