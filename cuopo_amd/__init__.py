@@ -109,6 +109,15 @@ SIGNATURES = {
     "opo_power_history": (_i, [_p, _p, _i]),
     "opo_set_tiles": (_i, [_p, _i, _i]),
     "opo_use_graph": (_i, [_p, _i]),
+    "opo_linear_factors": (_i, [_p, _p, _i, _i, _p]),
+    "opo_set_xcd_pack": (_i, [_p, _i]),
+    "opo_autotune": (_i, [_p, _i]),
+    "opo_get_tuning": (_i, [_p, _p]),
+    "opo_set_tuning": (_i, [_p, _p]),
+    "opo_batch_autotune": (_i, [_p, _i]),
+    "opo_batch_get_tuning": (_i, [_p, _p, _i]),
+    "opo_batch_set_tuning": (_i, [_p, _p]),
+    "opo_batch_time_stages": (_i, [_p, _i, _i, C.POINTER(_d), C.POINTER(_d)]),
     "opo_last_run_ms": (_i, [_p, C.POINTER(_d)]),
     "opo_time_stages": (_i, [_p, _i, C.POINTER(_d), C.POINTER(_d)]),
     "opo_copy_probe": (_i, [_sz, _i, _i, C.POINTER(_d), C.POINTER(_d)]),
@@ -137,6 +146,15 @@ SIGNATURES = {
 }
 
 _lib = None
+
+
+class Tuning(C.Structure):
+    """opo_tuning (include/opo.h)."""
+    _fields_ = [(n, C.c_int) for n in ("N", "count", "col_threads", "row_threads", "xcd_pack", "phase_mode",
+                                       "col_tile", "row_tile", "factored")] + [("slice_us", C.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 def lib():
@@ -335,6 +353,24 @@ class Solver:
     def use_graph(self, on=True):
         _check(lib().opo_use_graph(self._h, 1 if on else 0))
 
+    def set_xcd_pack(self, on=True):
+        _check(lib().opo_set_xcd_pack(self._h, 1 if on else 0))
+
+    def autotune(self, iters=10):
+        """Pick tiles / XCD packing / phase source by timing them on the
+        uploaded instance (fields are restored afterwards)."""
+        _check(lib().opo_autotune(self._h, iters))
+        return self.get_tuning()
+
+    def get_tuning(self):
+        t = Tuning()
+        _check(lib().opo_get_tuning(self._h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, col_threads=0, row_threads=0, xcd_pack=-1, phase_mode=-1):
+        t = Tuning(col_threads=col_threads, row_threads=row_threads, xcd_pack=xcd_pack, phase_mode=phase_mode)
+        _check(lib().opo_set_tuning(self._h, C.byref(t)))
+
     def last_run_ms(self):
         v = C.c_double()
         _check(lib().opo_last_run_ms(self._h, C.byref(v)))
@@ -428,6 +464,30 @@ class Batch:
         v = C.c_double()
         _check(lib().opo_batch_last_run_ms(self._h, C.byref(v)))
         return v.value
+
+    def autotune(self, iters=10):
+        """Pick tiles / XCD packing / phase source per grid size by timing
+        them on the uploaded instances (fields are restored afterwards)."""
+        _check(lib().opo_batch_autotune(self._h, iters))
+        return self.get_tuning()
+
+    def get_tuning(self):
+        n = lib().opo_batch_get_tuning(self._h, None, 0)
+        if n < 0:
+            _check(n)
+        arr = (Tuning * max(n, 1))()
+        lib().opo_batch_get_tuning(self._h, C.cast(arr, C.c_void_p), n)
+        return [arr[k].as_dict() for k in range(n)]
+
+    def set_tuning(self, N, col_threads=0, row_threads=0, xcd_pack=-1, phase_mode=-1):
+        t = Tuning(N=N, col_threads=col_threads, row_threads=row_threads, xcd_pack=xcd_pack,
+                   phase_mode=phase_mode)
+        _check(lib().opo_batch_set_tuning(self._h, C.byref(t)))
+
+    def time_stages(self, N, iters=20):
+        a, b = C.c_double(), C.c_double()
+        _check(lib().opo_batch_time_stages(self._h, N, iters, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
 
 def setup_instance(args: Args, precision=FP64, seed=1, noise_sigma=1e-20, device=0):

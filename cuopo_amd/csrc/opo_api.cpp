@@ -74,6 +74,8 @@ struct DevMem {
 struct Instance {
   opo_params prm{};
   void *state = nullptr, *lin = nullptr, *gdd = nullptr, *eom = nullptr, *pump = nullptr, *snap = nullptr;
+  void *lfac = nullptr;  // factorised linear phase (used when use_fac)
+  bool use_fac = false;
   void *save[3] = {nullptr, nullptr, nullptr};
   double *nl = nullptr;
   long long pump_len = 0;
@@ -82,6 +84,7 @@ struct Instance {
 
 struct Group {
   int N = 0, N1 = 0, N2 = 0, NF = 0, fp32 = 0, lobits = 0;
+  int xl_bits = 0;  // cross-term table split of the factorised linear phase
   std::vector<int> members;
   std::vector<Instance> inst;
   void *tw1 = nullptr, *tw2 = nullptr, *twlo = nullptr, *twhi = nullptr;
@@ -90,6 +93,10 @@ struct Group {
   double *d_partials = nullptr, *d_pref = nullptr, *d_pfirst = nullptr, *d_rprev = nullptr, *d_hist = nullptr;
   int hist_len = 0;
   int ct = 0, rt = 0;  // column tile (columns per block), row tile (rows per block)
+  int col_threads = 0, row_threads = 0;  // per-group overrides (0 = engine setting)
+  int xcd_pack = -1;                      // per-group override (-1 = engine setting)
+  int phase_mode = -1;                    // linear phase: -1 auto, 0 full table, 1 factorised
+  double tuned_us = 0;                    // slice time of the autotuner's choice (0 = not tuned)
   bool any_gdd = false;
   std::vector<void *> allocs;
   hipStream_t gs = nullptr;      // the group's own stream (batches with several grid sizes)
@@ -110,6 +117,7 @@ struct Engine {
   float last_ms = 0;
   int last_rt0 = 0, last_nrt = 0;
   int col_threads = 0, row_threads = 0;
+  int xcd_pack = 0;
   double off_ratio = 1e-6;
   bool inst_dirty = true;
   std::vector<std::pair<int, int>> where;  // batch index -> (group, slot)
@@ -132,6 +140,16 @@ struct Engine {
 };
 
 size_t esize(const Group &g) { return g.fp32 ? 8 : 16; }
+
+// Linear-phase source of a group.  The factorised tables (opo_phase.cpp) save
+// the N-point table read per field and slice but cost extra loads and
+// products per sample: measured on gfx950 they win once a group carries about
+// 2^19 samples per field (bandwidth-bound launches) and lose on smaller,
+// latency-bound ones.  The autotuner overrides the choice per group.
+bool group_factored(const Group &g) {
+  if (g.phase_mode >= 0) return g.phase_mode == 1;
+  return (long long)g.members.size() * g.N >= (1LL << 19);
+}
 
 int dalloc(Group &g, void **p, size_t bytes) {
   if (bytes == 0) { *p = nullptr; return OPO_OK; }
@@ -182,7 +200,7 @@ int validate(const opo_params *p) {
 
 // LDS bytes of one column-stage / row-stage block (must match the launchers)
 size_t col_lds(const Group &g, int ct) { return ((size_t)g.N1 + 2 * (size_t)g.NF * ct * g.N1) * esize(g) + 512; }
-size_t row_lds(const Group &g, int rt) { return ((size_t)g.N2 + 2 * (size_t)rt * g.N2) * esize(g); }
+size_t row_lds(const Group &g, int rt) { return 2 * (size_t)rt * (g.N2 + g.N2 / 8) * esize(g); }
 constexpr size_t kMaxLds = 160 * 1024;
 
 int floor_pow2(int x) {
@@ -197,8 +215,9 @@ int floor_pow2(int x) {
 void pick_tiles(Engine &E, Group &g) {
   const int T1 = g.N1 / 8, T2 = g.N2 / 8;
   // default column tile: one 128-byte line of columns (8 FP64 / 16 FP32)
-  int ct = E.col_threads ? std::max(1, E.col_threads / T1) : (int)(128 / esize(g));
-  int rtn = E.row_threads ? E.row_threads : 128;
+  const int colt = g.col_threads ? g.col_threads : E.col_threads;
+  int ct = colt ? std::max(1, colt / T1) : (int)(128 / esize(g));
+  int rtn = g.row_threads ? g.row_threads : (E.row_threads ? E.row_threads : 128);
   ct = std::min(floor_pow2(ct), g.N2);
   while (ct > 1 && g.NF * ct * T1 > 512) ct >>= 1;
   g.ct = ct;
@@ -230,6 +249,7 @@ int make_group(Engine &E, Group &g) {
   const int row_max = g.fp32 ? 4096 : 2048;
   while (g.N2 > row_max) { g.N2 >>= 1; g.N1 <<= 1; }
   g.lobits = (L + 1) / 2;
+  g.xl_bits = std::max(0, (L - 3 + 1) / 2);  // cross-term tables cover e < N/8
   const size_t es = esize(g);
   RCHK(dalloc(g, &g.tw1, g.N1 * es));
   RCHK(dalloc(g, &g.tw2, g.N2 * es));
@@ -258,7 +278,6 @@ int make_group(Engine &E, Group &g) {
   for (int k = 0; k < n; ++k) {
     Instance &I = g.inst[k];
     RCHK(dalloc(g, &I.state, (size_t)g.NF * g.N * es));
-    RCHK(dalloc(g, &I.lin, (size_t)g.NF * g.N * es));
     RCHK(dalloc(g, (void **)&I.nl, (size_t)E.NZ * 16 * sizeof(double)));
   }
   pick_tiles(E, g);
@@ -291,28 +310,32 @@ int upload_instance(Engine &E, Group &g, int k, const double *w, const double *T
   // position in the four-step frequency layout of natural frequency index q
   auto perm = [&](int q) { return (size_t)(g.N2 * (q % N1) + q / N1); };
 
-  // linear operator, cwes3.cuh:112-125 / cwes2.cuh:108-118 with z = dz (Q8); 1/N of CUFFTscale folded in
-  std::vector<cd> lin((size_t)NF * N);
-  double D[3], b2[3], b3[3], al[3];
-  if (E.three_eqs) {
-    D[0] = (1 / p.vp - 1 / p.vi); D[1] = (1 / p.vs - 1 / p.vs); D[2] = (1 / p.vi - 1 / p.vs);
-    b2[0] = p.b2p; b2[1] = p.b2s; b2[2] = p.b2i; b3[0] = p.b3p; b3[1] = p.b3s; b3[2] = p.b3i;
-    al[0] = p.alpha_crp; al[1] = p.alpha_crs; al[2] = p.alpha_cri;
-  } else {
-    D[0] = (1 / p.vp - 1 / p.vs); D[1] = (1 / p.vs - 1 / p.vs);
-    b2[0] = p.b2p; b2[1] = p.b2s; b3[0] = p.b3p; b3[1] = p.b3s;
-    al[0] = p.alpha_crp; al[1] = p.alpha_crs;
-  }
-  const double z = p.dz;
-  for (int f = 0; f < NF; ++f) {
-    const double atten = std::exp(-0.5 * al[f] * z);
-    for (int q = 0; q < N; ++q) {
-      const double wq = w[q];
-      const double ph = z * wq * (D[f] + 0.5 * wq * b2[f] + wq * wq * b3[f] / 6);
-      lin[(size_t)f * N + perm(q)] = cd(std::cos(ph), std::sin(ph)) * (atten / N);
+  // linear operator, cwes3.cuh:112-125 / cwes2.cuh:108-118 with z = dz (Q8); 1/N of CUFFTscale folded in.
+  // The full N-point table per field always; on the reference's grid also the
+  // factorised tables (opo_phase.cpp).  group_factored picks one per group.
+  {
+    std::vector<double> fac((size_t)2 * NF * opo::lin_factor_len(N, N1, g.xl_bits));
+    I.use_fac = opo::lin_factors(p, w, N1, g.xl_bits, fac.data()) == 1;
+    if (I.use_fac) {
+      if (!I.lfac) RCHK(dalloc(g, &I.lfac, fac.size() / 2 * es));
+      RCHK(upload_c(E, g, I.lfac, reinterpret_cast<const cd *>(fac.data()), fac.size() / 2));
     }
+    std::vector<cd> lin((size_t)NF * N);
+    double D[3], b2[3], b3[3], al[3];
+    opo::lin_coeffs(p, D, b2, b3, al);
+    const double z = p.dz;
+    for (int f = 0; f < NF; ++f) {
+      const double atten = std::exp(-0.5 * al[f] * z);
+      for (int q = 0; q < N; ++q) {
+        const double wq = w[q];
+        const double ph = z * wq * (D[f] + 0.5 * wq * b2[f] + wq * wq * b3[f] / 6);
+        lin[(size_t)f * N + perm(q)] = cd(std::cos(ph), std::sin(ph)) * (atten / N);
+      }
+    }
+    if (!I.lin) RCHK(dalloc(g, &I.lin, lin.size() * es));
+    RCHK(upload_c(E, g, I.lin, lin.data(), lin.size()));
+    E.inst_dirty = true;
   }
-  RCHK(upload_c(E, g, I.lin, lin.data(), lin.size()));
 
   // GDD compensation, cuOPO.cu:521-553 (AddGDD functions.cuh:156-169; AddGDDTOD :173-186)
   if (g.any_gdd) {
@@ -399,7 +422,7 @@ int sync_insts(Engine &E) {
       const opo_params &p = I.prm;
       OpoInst &d = h[k];
       std::memset(&d, 0, sizeof(d));
-      d.state = I.state; d.lin = I.lin; d.gdd = I.gdd; d.eom = p.use_eom ? I.eom : nullptr; d.nl = I.nl;
+      d.state = I.state; d.lin = I.lin; d.lfac = I.use_fac && group_factored(g) ? I.lfac : nullptr; d.gdd = I.gdd; d.eom = p.use_eom ? I.eom : nullptr; d.nl = I.nl;
       d.pump = I.pump; d.pump_len = I.pump_len;
       for (int f = 0; f < 3; ++f) d.save[f] = E.save_enabled ? I.save[f] : nullptr;
       d.save_first = p.regime == OPO_CW ? p.NRT - p.Nrts : 0;
@@ -434,6 +457,7 @@ int emit_col(Engine &E, Group &g, unsigned ops, int load, int inv, int fwd, int 
   a.ops = ops; a.load_mask = load; a.inv_mask = inv; a.fwd_mask = fwd; a.store_mask = store;
   a.s_end = s_end; a.s_start = s_start; a.pump_off = pump_off; a.three_eqs = E.three_eqs;
   a.snap_every = snap_every;
+  a.xcd_pack = g.xcd_pack >= 0 ? g.xcd_pack : E.xcd_pack;
   const int threads_ct = g.ct;
   int e = opo::launch_col(g.fp32, g.N1, g.NF, threads_ct, (int)g.members.size(), g.d_insts, a, g.cur, e0, e1);
   if (e != 0) return fail(OPO_ERR_HIP, std::string("col_stage launch: ") + hipGetErrorString((hipError_t)e));
@@ -448,6 +472,8 @@ int emit_row(Engine &E, Group &g, bool gdd, int mask, hipEvent_t e0 = nullptr, h
   for (int f = 0; f < g.NF; ++f)
     if ((mask >> f) & 1) fl[nf++] = f;
   a.nf = nf; a.f0 = fl[0]; a.f1 = fl[1]; a.f2 = fl[2];
+  a.xcd_pack = g.xcd_pack >= 0 ? g.xcd_pack : E.xcd_pack;
+  a.xl_bits = g.xl_bits;
   int e = opo::launch_row(g.fp32, g.N2, g.NF, g.rt, (int)g.members.size(), 0, g.d_insts, a, g.cur, e0, e1);
   if (e != 0) return fail(OPO_ERR_HIP, std::string("row_stage launch: ") + hipGetErrorString((hipError_t)e));
   return OPO_OK;
@@ -771,6 +797,13 @@ int opo_set_tiles(opo_solver *s, int col_threads, int row_threads) {
   return OPO_OK;
 }
 
+int opo_set_xcd_pack(opo_solver *s, int enable) {
+  if (!s) return fail(OPO_ERR_INVALID, "null solver");
+  s->E.xcd_pack = enable ? 1 : 0;
+  s->E.graph_key = -1;
+  return OPO_OK;
+}
+
 int opo_use_graph(opo_solver *s, int enable) {
   if (!s) return fail(OPO_ERR_INVALID, "null solver");
   s->E.use_graph = enable != 0;
@@ -1078,6 +1111,149 @@ int opo_batch_power(opo_batch *b, double *out) {
     }
   }
   return OPO_OK;
+}
+
+namespace {
+
+void get_tuning(const Engine &E, const Group &g, opo_tuning *t) {
+  std::memset(t, 0, sizeof(*t));
+  t->N = g.N;
+  t->count = (int)g.members.size();
+  t->col_threads = g.col_threads ? g.col_threads : E.col_threads;
+  t->row_threads = g.row_threads ? g.row_threads : E.row_threads;
+  t->xcd_pack = g.xcd_pack >= 0 ? g.xcd_pack : E.xcd_pack;
+  t->phase_mode = g.phase_mode;
+  t->col_tile = g.ct;
+  t->row_tile = g.rt;
+  t->factored = 0;
+  for (const auto &I : g.inst) t->factored |= (I.use_fac && group_factored(g)) ? 1 : 0;
+  t->slice_us = g.tuned_us;
+}
+
+int set_tuning(Engine &E, Group &g, const opo_tuning *t) {
+  for (int v : {t->col_threads, t->row_threads})
+    if (v != 0 && v != 32 && v != 64 && v != 128 && v != 256 && v != 512)
+      return fail(OPO_ERR_INVALID, "tile threads in {0,32,64,128,256,512}");
+  if (t->phase_mode < -1 || t->phase_mode > 1) return fail(OPO_ERR_INVALID, "phase_mode in {-1,0,1}");
+  g.col_threads = t->col_threads;
+  g.row_threads = t->row_threads;
+  g.xcd_pack = t->xcd_pack < 0 ? -1 : (t->xcd_pack ? 1 : 0);
+  g.phase_mode = t->phase_mode;
+  g.tuned_us = 0;
+  pick_tiles(E, g);
+  RCHK(check_shapes(g));
+  E.graph_key = -1;
+  E.inst_dirty = true;
+  return OPO_OK;
+}
+
+// Time every candidate (column tile, row tile, XCD packing, phase source) of
+// each group with engine_time_stages and keep the fastest.  The stage runs
+// mutate the fields, so each group's state is saved before and restored after.
+int engine_autotune(Engine &E, int iters) {
+  for (auto &g : E.groups) {
+    for (auto &I : g.inst)
+      if (!I.uploaded) return fail(OPO_ERR_STATE, "autotune before upload");
+    const size_t bytes = (size_t)g.NF * g.N * esize(g);
+    const int n = (int)g.members.size();
+    void *bak = nullptr;
+    HIPCHK(hipMalloc(&bak, bytes * n));
+    std::unique_ptr<void, void (*)(void *)> guard(bak, [](void *p) { (void)hipFree(p); });
+    for (int k = 0; k < n; ++k)
+      HIPCHK(hipMemcpyAsync((char *)bak + bytes * k, g.inst[k].state, bytes, hipMemcpyDeviceToDevice, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+    opo_tuning base;
+    get_tuning(E, g, &base);
+    const int T1 = g.N1 / 8;
+    const int ct0 = (int)(128 / esize(g));
+    std::vector<int> cols, rows = {64, 128, 256, 512};
+    for (int ct : {ct0 / 2, ct0, ct0 * 2}) {
+      const int th = ct * T1;
+      if (th >= 32 && th <= 512 && (th & (th - 1)) == 0) cols.push_back(th);
+    }
+    if (cols.empty()) cols.push_back(0);
+    std::vector<int> xcds = {0};
+    if ((long long)n * g.N <= (1LL << 16)) xcds.push_back(1);
+    bool can_fac = true;
+    for (auto &I : g.inst) can_fac &= I.use_fac;
+    std::vector<int> phases = {0};
+    if (can_fac) phases.push_back(1);
+    opo_tuning best = base;
+    double best_us = 1e30;
+    std::vector<std::pair<int, int>> seen;
+    for (int c : cols)
+      for (int r : rows)
+        for (int x : xcds)
+          for (int ph : phases) {
+            opo_tuning t = base;
+            t.col_threads = c; t.row_threads = r; t.xcd_pack = x; t.phase_mode = ph;
+            if (set_tuning(E, g, &t) != OPO_OK) continue;
+            RCHK(sync_insts(E));
+            double cu = 0, ru = 0;
+            RCHK(engine_time_stages(E, g, iters, &cu, &ru));
+            if (cu + ru < best_us) { best_us = cu + ru; best = t; }
+          }
+    RCHK(set_tuning(E, g, &best));
+    g.tuned_us = best_us;
+    for (int k = 0; k < n; ++k)
+      HIPCHK(hipMemcpyAsync(g.inst[k].state, (char *)bak + bytes * k, bytes, hipMemcpyDeviceToDevice, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+  }
+  RCHK(sync_insts(E));
+  return reset_counters(E, 0);
+}
+
+Group *group_of(Engine &E, int N) {
+  for (auto &g : E.groups)
+    if (g.N == N) return &g;
+  return nullptr;
+}
+
+}  // namespace
+
+int opo_autotune(opo_solver *s, int iters) {
+  if (!s || iters < 1) return fail(OPO_ERR_INVALID, "bad arguments");
+  HIPCHK(hipSetDevice(s->E.device));
+  return engine_autotune(s->E, iters);
+}
+
+int opo_get_tuning(opo_solver *s, opo_tuning *out) {
+  if (!s || !out) return fail(OPO_ERR_INVALID, "null");
+  get_tuning(s->E, s->E.groups[0], out);
+  return OPO_OK;
+}
+
+int opo_set_tuning(opo_solver *s, const opo_tuning *t) {
+  if (!s || !t) return fail(OPO_ERR_INVALID, "null");
+  return set_tuning(s->E, s->E.groups[0], t);
+}
+
+int opo_batch_autotune(opo_batch *b, int iters) {
+  if (!b || iters < 1) return fail(OPO_ERR_INVALID, "bad arguments");
+  HIPCHK(hipSetDevice(b->E.device));
+  return engine_autotune(b->E, iters);
+}
+
+int opo_batch_get_tuning(opo_batch *b, opo_tuning *out, int max) {
+  if (!b || (!out && max > 0)) return fail(OPO_ERR_INVALID, "null");
+  const int n = (int)b->E.groups.size();
+  for (int k = 0; k < n && k < max; ++k) get_tuning(b->E, b->E.groups[k], out + k);
+  return n;
+}
+
+int opo_batch_set_tuning(opo_batch *b, const opo_tuning *t) {
+  if (!b || !t) return fail(OPO_ERR_INVALID, "null");
+  Group *g = group_of(b->E, t->N);
+  if (!g) return fail(OPO_ERR_INVALID, "no group with that N");
+  return set_tuning(b->E, *g, t);
+}
+
+int opo_batch_time_stages(opo_batch *b, int N, int iters, double *col_us, double *row_us) {
+  if (!b || iters < 1) return fail(OPO_ERR_INVALID, "bad arguments");
+  HIPCHK(hipSetDevice(b->E.device));
+  for (auto &g : b->E.groups)
+    if (g.N == N) return engine_time_stages(b->E, g, iters, col_us, row_us);
+  return fail(OPO_ERR_INVALID, "no group with that N");
 }
 
 int opo_batch_last_run_ms(opo_batch *b, double *ms) {

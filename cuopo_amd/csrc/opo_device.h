@@ -68,82 +68,108 @@ template <int R, int S, typename V> __device__ __forceinline__ void dftr(V* u) {
   else if constexpr (R == 2) dft2<S>(u);
 }
 
-// One Stockham pass (radix R, stride P) for a team of M/8 threads.  On entry
-// v[m] holds x[t + m*M/8]; outputs go to LDS element (i/P)*P*R + k + j*P.
-// Team element e lives at buf[gbase + e*es].
-template <int M, int R, int P, int S, typename V>
-__device__ __forceinline__ void stockham_pass(const V* v, V* buf, int t, int gbase, int es, const V* tw) {
+// Team FFT of length M: a team of M/8 threads, each holding 8 samples in
+// registers, runs radix-8 Stockham passes (a radix-2 or -4 pass last when
+// log2 M is not a multiple of 3).  On entry v[m] = x[t + m*M/8]; on exit
+// v[m] = X[t + m*M/8].  Passes exchange through LDS; the last pass's outputs
+// already sit in the natural distribution (output i + j*M/R belongs to the
+// thread that read input i + j*M/R), so it runs in registers and an FFT of
+// npasses(M) passes costs npasses(M) - 1 exchanges.
+//
+// Team element e lives at buf[gbase + lidx(e)], lidx(e) = e*es (PAD false,
+// column tiles: es interleaved columns) or e + e/8 (PAD true, contiguous rows:
+// one pad slot per 8 samples makes the stride-8 writes of the first pass hit
+// distinct banks).  The LDS region holds two ping-pong buffers `bufstride`
+// elements apart; `ph` (block-uniform) alternates them across every exchange
+// of the kernel, so one barrier per exchange suffices: a buffer is rewritten
+// only after the barrier that follows the next exchange's writes.  `on` (per
+// thread) skips the arithmetic of threads whose data is not transformed while
+// every thread still reaches the barriers.
+//
+// Twiddles e^{+2 pi i e/M} come from `tws`: with TREG false an M-point root
+// table (LDS) indexed by e; with TREG true this thread's register set filled
+// by load_twiddles, indexed by (pass, butterfly, leg).
+
+template <bool PAD> __device__ __forceinline__ int lidx(int e, int es) { return PAD ? e + (e >> 3) : e * es; }
+
+// per-thread twiddles of passes 1 .. Q-1 (pass 0 has none)
+constexpr int tw_off(int M, int Q) {
+  return Q <= 1 ? 0 : tw_off(M, Q - 1) + (radix_at(M, Q - 1) - 1) * (8 / radix_at(M, Q - 1));
+}
+constexpr int tw_count(int M) { return tw_off(M, npasses(M)); }
+
+template <int M, int Q, int S, bool PAD, bool TREG, typename V>
+__device__ __forceinline__ void fft_pass(V *v, bool on, V *buf0, int &ph, int t, int gbase, int es, int bufstride,
+                                         const V *tws) {
+  constexpr int R = radix_at(M, Q);
+  constexpr int P = p_at(Q);
   constexpr int T = M / 8;
   constexpr int NB = 8 / R;
+  constexpr bool LAST = Q == npasses(M) - 1;
+  constexpr int TOFF = tw_off(M, Q);  // constant: keeps the register set statically indexed
+  V *buf = buf0 + (ph & 1) * bufstride;
+  if (on) {
 #pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    const int i = t + b * T;
-    const int k = i & (P - 1);
-    V u[R];
+    for (int b = 0; b < NB; ++b) {
+      const int i = t + b * T;
+      const int k = i & (P - 1);
+      V u[R];
 #pragma unroll
-    for (int j = 0; j < R; ++j) u[j] = v[b + j * NB];
-    if constexpr (P > 1) {
+      for (int j = 0; j < R; ++j) u[j] = v[b + j * NB];
+      if constexpr (P > 1) {
 #pragma unroll
-      for (int j = 1; j < R; ++j) u[j] = cmul(u[j], tw_sign<S>(tw[j * k * (M / (P * R))]));
-    }
-    dftr<R, S>(u);
-    const int base = (i / P) * P * R + k;
-#pragma unroll
-    for (int j = 0; j < R; ++j) buf[gbase + (base + j * P) * es] = u[j];
-  }
-}
-
-// Team FFT of length M over NFX fields at once (field f transformed when bit
-// f of `mask` is set; the mask is block-uniform).  v[f][m] = x_f[t + m*M/8]
-// on entry, X_f[t + m*M/8] on exit.  Field f's element e lives at
-// lds[f*fstride + gbase + e*es]; lds holds two ping-pong buffers `bufstride`
-// elements apart, and `ph` (block-uniform) alternates them across every pass
-// of the kernel, so one barrier per pass suffices: a buffer is rewritten only
-// after the barrier that follows the next pass's writes.  `tw` points to the
-// M-point root table in LDS.  Every thread of the block must call this.
-template <int M, int S, int NFX, int Q = 0, typename V>
-__device__ __forceinline__ void team_fft(V (*v)[8], int mask, V* lds, int& ph, int t, int gbase, int es,
-                                         int fstride, int bufstride, const V* tw) {
-  if constexpr (Q < npasses(M)) {
-    constexpr int R = radix_at(M, Q);
-    constexpr int P = p_at(Q);
-    constexpr int T = M / 8;
-    V* buf = lds + (ph & 1) * bufstride;
-#pragma unroll
-    for (int f = 0; f < NFX; ++f)
-      if ((mask >> f) & 1) stockham_pass<M, R, P, S>(v[f], buf + f * fstride, t, gbase, es, tw);
-    __syncthreads();
-#pragma unroll
-    for (int f = 0; f < NFX; ++f)
-      if ((mask >> f) & 1) {
-#pragma unroll
-        for (int m = 0; m < 8; ++m) v[f][m] = buf[f * fstride + gbase + (t + m * T) * es];
+        for (int j = 1; j < R; ++j) {
+          if constexpr (TREG) u[j] = cmul(u[j], tw_sign<S>(tws[TOFF + b * (R - 1) + j - 1]));
+          else u[j] = cmul(u[j], tw_sign<S>(tws[j * k * (M / (P * R))]));
+        }
       }
-    ++ph;
-    team_fft<M, S, NFX, Q + 1>(v, mask, lds, ph, t, gbase, es, fstride, bufstride, tw);
+      dftr<R, S>(u);
+      if constexpr (LAST) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[b + j * NB] = u[j];
+      } else {
+        const int base = (i / P) * P * R + k;
+#pragma unroll
+        for (int j = 0; j < R; ++j) buf[gbase + lidx<PAD>(base + j * P, es)] = u[j];
+      }
+    }
   }
-}
-
-// Single-field variant: each thread owns one field (its wave's field); `on`
-// (per thread) skips the work of threads whose field is not transformed,
-// while every thread still reaches the barriers.  `buf0` is this thread's
-// field region in buffer 0; buffer 1 is `bufstride` elements further.
-template <int M, int S, int Q = 0, typename V>
-__device__ __forceinline__ void team_fft1(V* v, bool on, V* buf0, int& ph, int t, int gbase, int es, int bufstride,
-                                          const V* tw) {
-  if constexpr (Q < npasses(M)) {
-    constexpr int R = radix_at(M, Q);
-    constexpr int P = p_at(Q);
-    constexpr int T = M / 8;
-    V* buf = buf0 + (ph & 1) * bufstride;
-    if (on) stockham_pass<M, R, P, S>(v, buf, t, gbase, es, tw);
+  if constexpr (!LAST) {
     __syncthreads();
     if (on) {
 #pragma unroll
-      for (int m = 0; m < 8; ++m) v[m] = buf[gbase + (t + m * T) * es];
+      for (int m = 0; m < 8; ++m) v[m] = buf[gbase + lidx<PAD>(t + m * T, es)];
     }
     ++ph;
-    team_fft1<M, S, Q + 1>(v, on, buf0, ph, t, gbase, es, bufstride, tw);
+  }
+}
+
+template <int M, int S, bool PAD, bool TREG, int Q = 0, typename V>
+__device__ __forceinline__ void team_fft(V *v, bool on, V *buf0, int &ph, int t, int gbase, int es, int bufstride,
+                                         const V *tws) {
+  if constexpr (Q < npasses(M)) {
+    fft_pass<M, Q, S, PAD, TREG>(v, on, buf0, ph, t, gbase, es, bufstride, tws);
+    team_fft<M, S, PAD, TREG, Q + 1>(v, on, buf0, ph, t, gbase, es, bufstride, tws);
+  }
+}
+
+// This thread's twiddles of every pass, from the M-point root table `g`
+// (global memory), into treg[tw_count(M)]; both transform directions use them.
+template <int M, int Q = 1, typename V>
+__device__ __forceinline__ void load_twiddles(V *treg, int t, const V *__restrict__ g) {
+  if constexpr (Q < npasses(M)) {
+    constexpr int R = radix_at(M, Q);
+    constexpr int P = p_at(Q);
+    constexpr int T = M / 8;
+    constexpr int NB = 8 / R;
+    constexpr int TOFF = tw_off(M, Q);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int k = (t + b * T) & (P - 1);
+#pragma unroll
+      for (int j = 1; j < R; ++j) treg[TOFF + b * (R - 1) + j - 1] = g[j * k * (M / (P * R))];
+    }
+    load_twiddles<M, Q + 1>(treg, t, g);
   }
 }
 

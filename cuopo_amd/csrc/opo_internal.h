@@ -11,7 +11,10 @@
 // are stored in that permuted order.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
+
+#include "../../include/opo.h"
 
 namespace opo {
 
@@ -30,7 +33,8 @@ enum ColOp : unsigned {
 // Per-instance descriptor (device memory, one per instance of a group).
 struct OpoInst {
   void *state;             // NF*N complex
-  const void *lin;         // NF*N complex: e^{i phase}*atten/N, permuted (p, s, i)
+  const void *lin;         // NF*N complex: e^{i phase}*atten/N, permuted (p, s, i); null when lfac is used
+  const void *lfac;        // factorised phase: per field Z (8*N1), S (N2), Xhi (N/8 >> L), Xlo (2^L); or null
   const void *gdd;         // 2N complex (signal, idler): e^{i w^2 GDD/2}/N, permuted; null if unused
   const void *eom;         // N complex: e^{i m sin(2 pi fpm T)}, time order; null if unused
   const double *nl;        // NZ*16 doubles: 8 complex FP64 coefficients per slice
@@ -71,6 +75,7 @@ struct ColArgs {
   int s_end, s_start, pump_off;
   int three_eqs;
   int snap_every;    // OP_SNAP fires when (rt + 1) % snap_every == 0
+  int xcd_pack;      // 1: grid is 8x wider and only blocks with blockIdx.x % 8 == 0 work
 };
 
 struct RowArgs {
@@ -78,6 +83,8 @@ struct RowArgs {
   int N1, log2rt;
   int nf, f0, f1, f2;
   int use_gdd;
+  int xcd_pack;      // as in ColArgs
+  int xl_bits;       // L of the cross-term tables of OpoInst::lfac
 };
 
 struct TickArgs {
@@ -97,6 +104,12 @@ int launch_row(int fp32, int N2, int NF, int rt, int count, int tiles_y, const O
                const RowArgs &a, void *stream, void *ev0 = nullptr, void *ev1 = nullptr);
 // streaming copy of n 16-byte elements (bandwidth probe / counter calibration)
 int launch_copy(const void *src, void *dst, long long n, void *stream, void *ev0, void *ev1);
+// linear-step phase coefficients (returns NF) and its factorisation (host)
+int lin_coeffs(const opo_params &p, double D[3], double b2[3], double b3[3], double al[3]);
+bool affine_grid(const double *w, int N, double &a0, double &a1, double &s);
+size_t lin_factor_len(int N, int N1, int xl_bits);
+int lin_factors(const opo_params &p, const double *w, int N1, int xl_bits, double *out);
+
 int launch_tick(OpoInst *insts, const TickArgs &a, void *stream);
 
 }  // namespace opo

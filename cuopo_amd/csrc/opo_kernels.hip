@@ -12,6 +12,7 @@
 //                 1/N), inverse row FFT.
 // Each stage reads and writes every field once with coalesced 16-byte
 // accesses; the sub-FFTs run in registers + LDS (team_fft).
+
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
 
@@ -42,15 +43,31 @@ template <typename V> __device__ __forceinline__ V pick(int f, V a, V b, V c) {
   return f == 0 ? a : r;
 }
 
+// Coefficients of one nonlinear half step (slice s), read once per block at
+// kernel start so their latency overlaps the field loads.
+template <int NF, typename V> struct NlCoef {
+  V c[NF == 3 ? 3 : 6];
+  decltype(V::x) kp, ks, h;
+};
+
+template <int NF, typename V> __device__ __forceinline__ NlCoef<NF, V> load_nl(const OpoInst *I, int s) {
+  NlCoef<NF, V> r;
+  const double *q = I->nl + 16 * s;
+#pragma unroll
+  for (int j = 0; j < (NF == 3 ? 3 : 6); ++j) r.c[j] = ld_c<V>(q + 2 * j);
+  using R = decltype(V::x);
+  r.kp = (R)I->kp; r.ks = (R)I->ks; r.h = (R)I->h;
+  return r;
+}
+
 // Pointwise nonlinear half step on all fields of K samples.
 template <int NF, int K, typename V>
-__device__ __forceinline__ void nl_half_pt(V *xp, V *xs, V *xi, const OpoInst *I, int s) {
-  const double *q = I->nl + 16 * s;
+__device__ __forceinline__ void nl_half_pt(V *xp, V *xs, V *xi, const NlCoef<NF, V> &q) {
   if constexpr (NF == 3) {
     // Closed form of cwes3.cuh:153-172: every stage evaluates dAdz on the
     // unchanged fields, so A += (h/6)(k1+2k2+2k3+k4) = A + C*P(A) with
     // C = i kappa (h/6)(E(z)+4E(z+dz/4)+E(z+dz/2)) precomputed in FP64.
-    const V Cp = ld_c<V>(q), Cs = ld_c<V>(q + 2), Ci = ld_c<V>(q + 4);
+    const V Cp = q.c[0], Cs = q.c[1], Ci = q.c[2];
 #pragma unroll
     for (int m = 0; m < K; ++m) {
       const V ap = xp[m], as = xs[m], ai = xi[m];
@@ -61,10 +78,10 @@ __device__ __forceinline__ void nl_half_pt(V *xp, V *xs, V *xi, const OpoInst *I
   } else {
     // cwes2.cuh:147-166: stages 2-4 evaluate dAdz on A + s*k (s = 0.5, 0.5, 1).
     using R = decltype(V::x);
-    const V ep0 = ld_c<V>(q), ep1 = ld_c<V>(q + 2), ep2 = ld_c<V>(q + 4);
-    const V em0 = ld_c<V>(q + 6), em1 = ld_c<V>(q + 8), em2 = ld_c<V>(q + 10);
-    const V ikp = mk<V>((R)0, (R)I->kp), iks = mk<V>((R)0, (R)I->ks);
-    const R h = (R)I->h;
+    const V ep0 = q.c[0], ep1 = q.c[1], ep2 = q.c[2];
+    const V em0 = q.c[3], em1 = q.c[4], em2 = q.c[5];
+    const V ikp = mk<V>((R)0, q.kp), iks = mk<V>((R)0, q.ks);
+    const R h = q.h;
 #pragma unroll
     for (int m = 0; m < K; ++m) {
       const V ap = xp[m], as = xs[m];
@@ -101,6 +118,10 @@ __global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ ins
 
   const OpoInst *I = insts + blockIdx.y;
   if (I->active_v == 0) return;  // block-uniform
+  // XCD packing: work-groups are dispatched round-robin over the 8 XCDs, so
+  // keeping only every 8th block puts the whole stage on one XCD (and its L2)
+  if (a.xcd_pack && (blockIdx.x & 7)) return;
+  const int bx = a.xcd_pack ? (int)(blockIdx.x >> 3) : (int)blockIdx.x;
 
   const int CT = 1 << a.log2ct;
   const int FT = CT * T1;
@@ -110,7 +131,7 @@ __global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ ins
   const int c = lt & (CT - 1);
   const int t = lt >> a.log2ct;
   const int N2 = a.N2;
-  const int n2 = blockIdx.x * CT + c;
+  const int n2 = bx * CT + c;
   const long long N = (long long)N1 * N2;
   const unsigned nmask = (unsigned)(N - 1);
   V *st = reinterpret_cast<V *>(I->state);
@@ -122,6 +143,10 @@ __global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ ins
   const int store_mask = MID ? (1 << NF) - 1 : a.store_mask;
   const bool any_fft = (inv_mask | fwd_mask) != 0;
 
+  const unsigned ops = MID ? (OP_NL_END | OP_NL_START) : a.ops;
+  NlCoef<NF, V> nle{}, nls{};
+  if (ops & OP_NL_END) nle = load_nl<NF, V>(I, a.s_end);
+  if (ops & OP_NL_START) nls = load_nl<NF, V>(I, a.s_start);
   if (any_fft) {
     const V *tw1 = reinterpret_cast<const V *>(a.tw1);
     for (int e = tid; e < N1; e += blockDim.x) ltw[e] = tw1[e];
@@ -152,19 +177,16 @@ __global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ ins
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = cmul(v[m], cconj(W[m]));
     }
-    team_fft1<N1, -1>(v, inv, mine0, ph, t, c, CT, bufstride, ltw);
+    team_fft<N1, -1, false, false>(v, inv, mine0, ph, t, c, CT, bufstride, ltw);
   }
-  // ---- all fields meet in LDS buffer B (the last pass wrote the
-  //      transformed ones there; the others are written now) ----
-  V *xb = lds + ((ph + 1) & 1) * bufstride;
-  if (!inv) {
+  // ---- all fields meet in the next LDS buffer ----
+  V *xb = lds + (ph & 1) * bufstride;
 #pragma unroll
-    for (int m = 0; m < 8; ++m) xb[f * fstride + c + (t + m * T1) * CT] = v[m];
-  }
+  for (int m = 0; m < 8; ++m) xb[f * fstride + c + (t + m * T1) * CT] = v[m];
   __syncthreads();
+  ++ph;
   // ---- pointwise program on time samples n = N2*(t + m*T1) + n2, in two
   //      chunks of 4 samples to bound the live registers ----
-  const unsigned ops = MID ? (OP_NL_END | OP_NL_START) : a.ops;
   int slot = -1, rp = 0;
   bool snap = false;
   if constexpr (!MID) {
@@ -185,7 +207,7 @@ __global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ ins
       xs[q] = xb[fstride + o];
       xi[q] = NF == 3 ? xb[2 * fstride + o] : mk<V>(0, 0);
     }
-    if (ops & OP_NL_END) nl_half_pt<NF, 4>(xp, xs, xi, I, a.s_end);
+    if (ops & OP_NL_END) nl_half_pt<NF, 4>(xp, xs, xi, nle);
     if constexpr (!MID) {
       if ((ops & OP_EOM) && I->eom) {
         const V *eom = reinterpret_cast<const V *>(I->eom);
@@ -237,7 +259,7 @@ __global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ ins
         for (int q = 0; q < 4; ++q) xp[q] = pin[off + (long long)N2 * (t + (hb + q) * T1) + n2];
       }
     }
-    if (ops & OP_NL_START) nl_half_pt<NF, 4>(xp, xs, xi, I, a.s_start);
+    if (ops & OP_NL_START) nl_half_pt<NF, 4>(xp, xs, xi, nls);
 #pragma unroll
     for (int q = 0; q < 4; ++q) v[hb + q] = pick(f, xp[q], xs[q], xi[q]);
   };
@@ -257,7 +279,7 @@ __global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ ins
         double sum = 0;
         const int nw = (blockDim.x + 63) >> 6;
         for (int w = 0; w < nw; ++w) sum += red[w];
-        I->partials[blockIdx.x] = sum;
+        I->partials[bx] = sum;
       }
     }
   }
@@ -265,7 +287,7 @@ __global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ ins
   // ---- forward column transform of this group's field ----
   const bool fw = (fwd_mask >> f) & 1;
   if (fwd_mask) {
-    team_fft1<N1, +1>(v, fw, mine0, ph, t, c, CT, bufstride, ltw);
+    team_fft<N1, +1, false, false>(v, fw, mine0, ph, t, c, CT, bufstride, ltw);
     if (fw) {
 #pragma unroll
       for (int m = 0; m < 8; ++m) v[m] = cmul(v[m], W[m]);
@@ -281,43 +303,66 @@ template <typename T, int N2, int NF>
 __global__ __launch_bounds__(512) void row_stage(const OpoInst *__restrict__ insts, RowArgs a) {
   using V = typename vec2<T>::type;
   constexpr int T2 = N2 / 8;
+  constexpr int NP2 = N2 + N2 / 8;  // padded row in LDS
   extern __shared__ unsigned char smem_raw[];
-  V *ltw = reinterpret_cast<V *>(smem_raw);  // N2-point root table
-  V *lds = ltw + N2;                         // 2 buffers x RT*N2
+  V *lds = reinterpret_cast<V *>(smem_raw);  // 2 buffers x RT padded rows
 
   const OpoInst *I = insts + blockIdx.y;
   if (I->active_v == 0) return;
+  if (a.xcd_pack && (blockIdx.x & 7)) return;  // see col_stage
+  const int bx = a.xcd_pack ? (int)(blockIdx.x >> 3) : (int)blockIdx.x;
 
   const int RT = 1 << a.log2rt;
   const int tiles_per_field = a.N1 >> a.log2rt;
-  const int fi = blockIdx.x / tiles_per_field;
+  const int fi = bx / tiles_per_field;
   const int f = fi == 0 ? a.f0 : (fi == 1 ? a.f1 : a.f2);
   const int tid = threadIdx.x;
   const int t = tid & (T2 - 1);
   const int r = tid / T2;
-  const int k1 = (blockIdx.x - fi * tiles_per_field) * RT + r;
+  const int k1 = (bx - fi * tiles_per_field) * RT + r;
   const long long N = (long long)a.N1 * N2;
   V *row = reinterpret_cast<V *>(I->state) + f * N + (long long)N2 * k1;
+
+  V v[8], ph_tab[8], treg[tw_count(N2) > 0 ? tw_count(N2) : 1];
+  const V *fac = reinterpret_cast<const V *>(I->lfac);
+  const bool use_fac = !a.use_gdd && fac;
   const V *tab = a.use_gdd ? reinterpret_cast<const V *>(I->gdd) + (f - 1) * N
                           : reinterpret_cast<const V *>(I->lin) + f * N;
-  tab += (long long)N2 * k1;
-  const V *tw2 = reinterpret_cast<const V *>(a.tw2);
-  for (int e = tid; e < N2; e += blockDim.x) ltw[e] = tw2[e];
-
-  V v[1][8];
-  V ph_tab[8];
+  // phase loads first (the factorised products only wait for these), then
+  // the row, then the twiddles
+  V xr = mk<V>(0, 0);
+  if (use_fac) {
+    // factorised phase Z[8 k1 + m] S[t + m T2] X(k1 t), X = Xhi Xlo
+    const int L = a.xl_bits;
+    const long long FB = 8LL * a.N1 + N2 + ((N / 8) >> L) + (1LL << L);
+    const V *F = fac + f * FB;
+    const V *S = F + 8 * a.N1;
+    const V *XH = S + N2;
+    const V *XL = XH + ((N / 8) >> L);
+    const unsigned e = (unsigned)k1 * (unsigned)t;
+    xr = cmul(XH[e >> L], XL[e & ((1u << L) - 1u)]);
 #pragma unroll
-  for (int m = 0; m < 8; ++m) v[0][m] = row[t + m * T2];
+    for (int m = 0; m < 8; ++m) ph_tab[m] = cmul(F[8 * k1 + m], S[t + m * T2]);
+  } else {
+    tab += (long long)N2 * k1;
 #pragma unroll
-  for (int m = 0; m < 8; ++m) ph_tab[m] = tab[t + m * T2];
+    for (int m = 0; m < 8; ++m) ph_tab[m] = tab[t + m * T2];
+  }
+#pragma unroll
+  for (int m = 0; m < 8; ++m) v[m] = row[t + m * T2];
+  load_twiddles<N2>(treg, t, reinterpret_cast<const V *>(a.tw2));
+  if (use_fac) {
+#pragma unroll
+    for (int m = 0; m < 8; ++m) ph_tab[m] = cmul(ph_tab[m], xr);
+  }
   int ph = 0;
-  const int bufstride = RT * N2;
-  team_fft<N2, +1, 1>(v, 1, lds, ph, t, r * N2, 1, 0, bufstride, ltw);
+  const int bufstride = RT * NP2;
+  team_fft<N2, +1, true, true>(v, true, lds, ph, t, r * NP2, 1, bufstride, treg);
 #pragma unroll
-  for (int m = 0; m < 8; ++m) v[0][m] = cmul(v[0][m], ph_tab[m]);
-  team_fft<N2, -1, 1>(v, 1, lds, ph, t, r * N2, 1, 0, bufstride, ltw);
+  for (int m = 0; m < 8; ++m) v[m] = cmul(v[m], ph_tab[m]);
+  team_fft<N2, -1, true, true>(v, true, lds, ph, t, r * NP2, 1, bufstride, treg);
 #pragma unroll
-  for (int m = 0; m < 8; ++m) row[t + m * T2] = v[0][m];
+  for (int m = 0; m < 8; ++m) row[t + m * T2] = v[m];
 }
 
 // One thread per instance: reduce the column-tile partials in order, record
@@ -373,7 +418,7 @@ static int launch_col_t(int ct, int count, const OpoInst *insts, const ColArgs &
   using V = typename vec2<T>::type;
   const int threads = NF * ct * (N1 / 8);
   const size_t lds = ((size_t)N1 + 2 * (size_t)NF * ct * N1) * sizeof(V) + 64 * sizeof(double);
-  dim3 grid(a.N2 / ct, count);
+  dim3 grid((a.N2 / ct) * (a.xcd_pack ? 8 : 1), count);
   const bool mid = a.ops == (OP_NL_END | OP_NL_START) && a.inv_mask == (1 << NF) - 1 &&
                    a.fwd_mask == (1 << NF) - 1 && a.load_mask == (1 << NF) - 1 && a.store_mask == (1 << NF) - 1;
   static bool big_lds[2] = {false, false};  // gfx950 has 160 KiB of LDS; opt in above 64 KiB once
@@ -393,8 +438,8 @@ static int launch_row_t(int rt, int count, int tiles_y, const OpoInst *insts, co
                         hipEvent_t e0, hipEvent_t e1) {
   using V = typename vec2<T>::type;
   const int threads = rt * (N2 / 8);
-  const size_t lds = ((size_t)N2 + 2 * (size_t)rt * N2) * sizeof(V);
-  dim3 grid((a.N1 / rt) * a.nf, count);
+  const size_t lds = 2 * (size_t)rt * (N2 + N2 / 8) * sizeof(V);
+  dim3 grid((a.N1 / rt) * a.nf * (a.xcd_pack ? 8 : 1), count);
   (void)tiles_y;
   static bool big_lds = false;
   if (lds > 65536 && !big_lds) {

@@ -108,6 +108,16 @@ void opo_noise(uint64_t seed, uint64_t stream, double sigma, double *A, size_t n
 /* Phase-matching temperature scan (functions.cuh:23-54). */
 double opo_find_pm_temp(int crystal, double lp, double ls, double Lambda, double Tmin, double Tmax,
                         int pm_mode);
+/* Factorised phase of the linear step (cwes3.cuh:112-125 / cwes2.cuh:108-118)
+ * that the row stage uses on the reference's frequency grid.  Per field:
+ * Z (8*N1), S (N/N1), Xhi ((N/8) >> xl_bits), Xlo (1 << xl_bits) complex
+ * values with, for frequency q = k1 + N1 k2, k2 = t + m N/(8 N1) (m < 8),
+ * e = k1 t:  e^{i phase_q} e^{-alpha dz/2}/N =
+ * Z[8 k1 + m] S[k2] Xhi[e >> xl_bits] Xlo[e & (2^xl_bits - 1)].
+ * Returns 1 (out filled), 0 (w is not the affine two-half grid or a
+ * third-order term is nonzero: the solver then uses the full phase table) or
+ * OPO_ERR_INVALID.  No reference counterpart. */
+int opo_linear_factors(const opo_params *p, const double *w, int N1, int xl_bits, double *out);
 /* Refractive index and dispersion of the crystal (ppln.cuh / spplt.cuh). */
 double opo_n(int crystal, double L, double T);
 double opo_group_velocity(int crystal, double L, double T);
@@ -142,6 +152,29 @@ int opo_power_history(opo_solver *s, double *out, int n);
  * {32,64,128,256,512}; 0 = default (128-byte column tiles, 128-thread rows).
  * No reference counterpart (autotuner hook). */
 int opo_set_tiles(opo_solver *s, int col_threads, int row_threads);
+/* Launch settings of the instances of one grid size (no reference
+ * counterpart).  Inputs: col_threads / row_threads (threads per field of a
+ * column tile / per row tile; 0 = default), xcd_pack (-1 default, 0, 1),
+ * phase_mode (-1 auto, 0 full phase table, 1 factorised tables).  Outputs of
+ * get: N, count, col_tile / row_tile (columns / rows per block), factored
+ * (the factorised phase is in use) and slice_us (col + row stage time of the
+ * autotuner's choice, 0 if not tuned). */
+typedef struct {
+  int N, count;
+  int col_threads, row_threads, xcd_pack, phase_mode;
+  int col_tile, row_tile, factored;
+  double slice_us;
+} opo_tuning;
+/* Times the candidate settings on the uploaded instances (iters launches of
+ * each stage per candidate) and keeps the fastest; the fields are restored
+ * and the round-trip counters reset afterwards. */
+int opo_autotune(opo_solver *s, int iters);
+int opo_get_tuning(opo_solver *s, opo_tuning *out);
+int opo_set_tuning(opo_solver *s, const opo_tuning *t);
+/* Launch every stage on one XCD (8x grid, 7 of 8 blocks exit at once), so a
+ * small instance's stage-to-stage data stays in that XCD's L2.  Autotuner
+ * knob; no reference counterpart. */
+int opo_set_xcd_pack(opo_solver *s, int enable);
 /* Capture one steady round trip in a hipGraph and replay it (default on).
  * No reference counterpart. */
 int opo_use_graph(opo_solver *s, int enable);
@@ -182,6 +215,15 @@ int opo_batch_power_history(opo_batch *b, int idx, double *out, int n);
 /* final sum |As|^2 per instance (count values) */
 int opo_batch_power(opo_batch *b, double *out);
 int opo_batch_last_run_ms(opo_batch *b, double *ms);
+/* Per-group launch settings (autotuner; no reference counterpart).  The
+ * instances of one grid size N form a group launched together. */
+int opo_batch_autotune(opo_batch *b, int iters);
+/* fills min(max, groups) entries, returns the number of groups */
+int opo_batch_get_tuning(opo_batch *b, opo_tuning *out, int max);
+/* applies t to the group with grid size t->N */
+int opo_batch_set_tuning(opo_batch *b, const opo_tuning *t);
+/* opo_time_stages for the group of grid size N. */
+int opo_batch_time_stages(opo_batch *b, int N, int iters, double *col_us, double *row_us);
 int opo_batch_set_tiles(opo_batch *b, int col_threads, int row_threads);
 int opo_batch_destroy(opo_batch *b);
 
