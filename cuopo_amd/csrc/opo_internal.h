@@ -76,6 +76,7 @@ struct ColArgs {
   int three_eqs;
   int snap_every;    // OP_SNAP fires when (rt + 1) % snap_every == 0
   int xcd_pack;      // 1: grid is 8x wider and only blocks with blockIdx.x % 8 == 0 work
+  int swz;           // 1: XCD-contiguous tile order (tiles narrower than a cache line)
 };
 
 struct RowArgs {

@@ -121,7 +121,11 @@ __global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ ins
   // XCD packing: work-groups are dispatched round-robin over the 8 XCDs, so
   // keeping only every 8th block puts the whole stage on one XCD (and its L2)
   if (a.xcd_pack && (blockIdx.x & 7)) return;
-  const int bx = a.xcd_pack ? (int)(blockIdx.x >> 3) : (int)blockIdx.x;
+  int bx = a.xcd_pack ? (int)(blockIdx.x >> 3) : (int)blockIdx.x;
+  // tile swizzle: block b runs on XCD b % 8, so tile (b % 8)*(tiles/8) + b/8
+  // gives each XCD a contiguous range of columns and the tiles that share a
+  // cache line (tiles narrower than 128 bytes) the same L2
+  if (a.swz) bx = (bx & 7) * (int)(gridDim.x >> 3) + (bx >> 3);
 
   const int CT = 1 << a.log2ct;
   const int FT = CT * T1;
@@ -185,6 +189,27 @@ __global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ ins
   for (int m = 0; m < 8; ++m) xb[f * fstride + c + (t + m * T1) * CT] = v[m];
   __syncthreads();
   ++ph;
+  if constexpr (MID) {
+    // Slice interior: the thread of field f updates every field at its
+    // samples m = f (mod NF), in place (each sample belongs to one thread),
+    // then every thread reads its own field back.
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      if (m % NF == f) {
+        const int o = c + (t + m * T1) * CT;
+        V xp[1] = {xb[o]}, xs[1] = {xb[fstride + o]};
+        V xi[1] = {NF == 3 ? xb[2 * fstride + o] : mk<V>(0, 0)};
+        nl_half_pt<NF, 1>(xp, xs, xi, nle);
+        nl_half_pt<NF, 1>(xp, xs, xi, nls);
+        xb[o] = xp[0];
+        xb[fstride + o] = xs[0];
+        if constexpr (NF == 3) xb[2 * fstride + o] = xi[0];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = xb[f * fstride + c + (t + m * T1) * CT];
+  } else {
   // ---- pointwise program on time samples n = N2*(t + m*T1) + n2, in two
   //      chunks of 4 samples to bound the live registers ----
   int slot = -1, rp = 0;
@@ -265,7 +290,7 @@ __global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ ins
   };
   chunk(std::integral_constant<int, 0>{});
   chunk(std::integral_constant<int, 4>{});
-  if constexpr (!MID) {
+  {
     if (ops & OP_POWER) {
       // deterministic block reduction: wave shuffle tree, then waves in order;
       // blocks of 32 threads occupy half a wave, so start the tree inside the block
@@ -283,6 +308,7 @@ __global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ ins
       }
     }
   }
+  }  // boundary / general path
 
   // ---- forward column transform of this group's field ----
   const bool fw = (fwd_mask >> f) & 1;
@@ -417,10 +443,10 @@ static int launch_col_t(int ct, int count, const OpoInst *insts, const ColArgs &
                         hipEvent_t e1) {
   using V = typename vec2<T>::type;
   const int threads = NF * ct * (N1 / 8);
-  const size_t lds = ((size_t)N1 + 2 * (size_t)NF * ct * N1) * sizeof(V) + 64 * sizeof(double);
   dim3 grid((a.N2 / ct) * (a.xcd_pack ? 8 : 1), count);
   const bool mid = a.ops == (OP_NL_END | OP_NL_START) && a.inv_mask == (1 << NF) - 1 &&
                    a.fwd_mask == (1 << NF) - 1 && a.load_mask == (1 << NF) - 1 && a.store_mask == (1 << NF) - 1;
+  const size_t lds = ((size_t)N1 + 2 * (size_t)NF * ct * N1) * sizeof(V) + 64 * sizeof(double);
   static bool big_lds[2] = {false, false};  // gfx950 has 160 KiB of LDS; opt in above 64 KiB once
   auto kern = mid ? &col_stage<T, N1, NF, true> : &col_stage<T, N1, NF, false>;
   if (lds > 65536 && !big_lds[mid]) {
