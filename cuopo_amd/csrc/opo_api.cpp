@@ -96,7 +96,6 @@ struct Group {
   int col_threads = 0, row_threads = 0;  // per-group overrides (0 = engine setting)
   int xcd_pack = -1;                      // per-group override (-1 = engine setting)
   int phase_mode = -1;                    // linear phase: -1 auto, 0 full table, 1 factorised
-  int col_lean = 0;                       // slice-interior column launches: 1 single-buffer variant
   double tuned_us = 0;                    // slice time of the autotuner's choice (0 = not tuned)
   bool any_gdd = false;
   std::vector<void *> allocs;
@@ -460,7 +459,6 @@ int emit_col(Engine &E, Group &g, unsigned ops, int load, int inv, int fwd, int 
   a.snap_every = snap_every;
   a.xcd_pack = g.xcd_pack >= 0 ? g.xcd_pack : E.xcd_pack;
   a.swz = !a.xcd_pack && g.ct * esize(g) < 128 && (g.N2 / g.ct) % 8 == 0;
-  a.lean = g.col_lean;
   const int threads_ct = g.ct;
   int e = opo::launch_col(g.fp32, g.N1, g.NF, threads_ct, (int)g.members.size(), g.d_insts, a, g.cur, e0, e1);
   if (e != 0) return fail(OPO_ERR_HIP, std::string("col_stage launch: ") + hipGetErrorString((hipError_t)e));
@@ -1126,7 +1124,6 @@ void get_tuning(const Engine &E, const Group &g, opo_tuning *t) {
   t->row_threads = g.row_threads ? g.row_threads : E.row_threads;
   t->xcd_pack = g.xcd_pack >= 0 ? g.xcd_pack : E.xcd_pack;
   t->phase_mode = g.phase_mode;
-  t->col_lean = g.col_lean;
   t->col_tile = g.ct;
   t->row_tile = g.rt;
   t->factored = 0;
@@ -1139,12 +1136,10 @@ int set_tuning(Engine &E, Group &g, const opo_tuning *t) {
     if (v != 0 && v != 32 && v != 64 && v != 128 && v != 256 && v != 512)
       return fail(OPO_ERR_INVALID, "tile threads in {0,32,64,128,256,512}");
   if (t->phase_mode < -1 || t->phase_mode > 1) return fail(OPO_ERR_INVALID, "phase_mode in {-1,0,1}");
-  if (t->col_lean < 0 || t->col_lean > 1) return fail(OPO_ERR_INVALID, "col_lean in {0,1}");
   g.col_threads = t->col_threads;
   g.row_threads = t->row_threads;
   g.xcd_pack = t->xcd_pack < 0 ? -1 : (t->xcd_pack ? 1 : 0);
   g.phase_mode = t->phase_mode;
-  g.col_lean = t->col_lean;
   g.tuned_us = 0;
   pick_tiles(E, g);
   RCHK(check_shapes(g));
@@ -1184,17 +1179,15 @@ int engine_autotune(Engine &E, int iters) {
     for (auto &I : g.inst) can_fac &= I.use_fac;
     std::vector<int> phases = {0};
     if (can_fac) phases.push_back(1);
-    const int leans[2] = {0, 1};
     opo_tuning best = base;
     double best_us = 1e30;
     std::vector<std::pair<int, int>> seen;
     for (int c : cols)
       for (int r : rows)
         for (int x : xcds)
-          for (int ph : phases)
-            for (int ln : leans) {
+          for (int ph : phases) {
             opo_tuning t = base;
-            t.col_threads = c; t.row_threads = r; t.xcd_pack = x; t.phase_mode = ph; t.col_lean = ln;
+            t.col_threads = c; t.row_threads = r; t.xcd_pack = x; t.phase_mode = ph;
             if (set_tuning(E, g, &t) != OPO_OK) continue;
             RCHK(sync_insts(E));
             double cu = 0, ru = 0;

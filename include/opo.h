@@ -155,15 +155,13 @@ int opo_set_tiles(opo_solver *s, int col_threads, int row_threads);
 /* Launch settings of the instances of one grid size (no reference
  * counterpart).  Inputs: col_threads / row_threads (threads per field of a
  * column tile / per row tile; 0 = default), xcd_pack (-1 default, 0, 1),
- * phase_mode (-1 auto, 0 full phase table, 1 factorised tables), col_lean
- * (slice-interior column launches: 0 double-buffered, 1 single-buffered with
- * fewer live registers, for more blocks per CU).  Outputs of
+ * phase_mode (-1 auto, 0 full phase table, 1 factorised tables).  Outputs of
  * get: N, count, col_tile / row_tile (columns / rows per block), factored
  * (the factorised phase is in use) and slice_us (col + row stage time of the
  * autotuner's choice, 0 if not tuned). */
 typedef struct {
   int N, count;
-  int col_threads, row_threads, xcd_pack, phase_mode, col_lean;
+  int col_threads, row_threads, xcd_pack, phase_mode;
   int col_tile, row_tile, factored;
   double slice_us;
 } opo_tuning;
