@@ -1,8 +1,11 @@
 """What the trained surrogate gains: the same sweep points run from noise
-(two independent seeds) and from the surrogate's warm start, FP64, batched,
-to the steady-state test.  Reports round trips per point, finish states and
-the steady-state power of the warm start against the cold start, next to the
-cold-vs-cold spread that bounds any warm-start accuracy claim.
+(two independent seeds), from the surrogate's warm start, and from the mean
+training target (the same seed for every point: the ablation that separates
+what the network adds from what any plausible seed at the right scale adds),
+FP64, batched, to the steady-state test.  Reports round trips per point,
+finish states and the steady-state power of each warm start against the cold
+start, next to the cold-vs-cold spread that bounds any warm-start accuracy
+claim.
 
     python -m cuopo_amd.ab_surrogate --weights cuopo_amd/data/surrogate.npz
 """
@@ -31,18 +34,21 @@ def run(weights, count=32, N=4096, NZ=25, max_rt=20000, seed=7, precision=FP64):
         pts.append((N, nth, delta, lcr, gdd, R))
     ap0 = [derive(sweep_args(*p[1:], N=N, NZ=NZ)).Ap0 for p in pts]
     warm = sur.warm_fields([p[1:] for p in pts], N, ap0, seed=seed)
+    mean = sur.warm_fields([p[1:] for p in pts], N, ap0, seed=seed, use_mean=True)
     out = {"meta": meta, "points": pts}
-    for name, kw in (("cold_a", dict(seed=1000)), ("cold_b", dict(seed=5000)), ("warm", dict(warm=warm))):
+    for name, kw in (("cold_a", dict(seed=1000)), ("cold_b", dict(seed=5000)), ("warm", dict(warm=warm)),
+                     ("mean", dict(warm=mean))):
         r = SW.run_batched(pts, NZ=NZ, max_rt=max_rt, precision=precision, **kw)
         out[name] = dict(rts=r["rts"].tolist(), states=r["states"].tolist(), power=r["power"].tolist(),
                          run_s=r["run_s"])
         r["batch"].close()
         print(name, "run_s", round(r["run_s"], 2), "states", np.bincount(r["states"] + 1, minlength=4).tolist(),
               flush=True)
-    ca, cb, w = (np.array(out[k]["power"]) for k in ("cold_a", "cold_b", "warm"))
-    sa, sb, sw = (np.array(out[k]["states"]) for k in ("cold_a", "cold_b", "warm"))
-    ra, rw = np.array(out["cold_a"]["rts"]), np.array(out["warm"]["rts"])
+    ca, cb, w, mn = (np.array(out[k]["power"]) for k in ("cold_a", "cold_b", "warm", "mean"))
+    sa, sb, sw, sm = (np.array(out[k]["states"]) for k in ("cold_a", "cold_b", "warm", "mean"))
+    ra, rw, rm = np.array(out["cold_a"]["rts"]), np.array(out["warm"]["rts"]), np.array(out["mean"]["rts"])
     ok = (sa == 1) & (sw == 1)
+    okm = (sa == 1) & (sm == 1)
     okc = (sa == 1) & (sb == 1)
     summ = dict(
         points=count, steady_cold=int((sa == 1).sum()), steady_warm=int((sw == 1).sum()),
@@ -50,11 +56,15 @@ def run(weights, count=32, N=4096, NZ=25, max_rt=20000, seed=7, precision=FP64):
         same_outcome=int((sa == sw).sum()),
         mean_rts_cold=float(np.mean(np.where(ra > 0, ra, max_rt))),
         mean_rts_warm=float(np.mean(np.where(rw > 0, rw, max_rt))),
+        mean_rts_mean_seed=float(np.mean(np.where(rm > 0, rm, max_rt))),
+        steady_mean_seed=int((sm == 1).sum()), off_mean_seed=int((sm == 2).sum()),
+        same_outcome_mean_seed=int((sa == sm).sum()),
         mean_rts_cold_steady=float(np.mean(ra[sa == 1])) if (sa == 1).any() else None,
         mean_rts_warm_on_cold_steady=float(np.mean(np.where(rw > 0, rw, max_rt)[sa == 1])) if (sa == 1).any() else None,
         rel_power_warm_vs_cold=(np.abs(w - ca) / ca)[ok].tolist(),
+        rel_power_mean_seed_vs_cold=(np.abs(mn - ca) / ca)[okm].tolist(),
         rel_power_cold_b_vs_cold_a=(np.abs(cb - ca) / ca)[okc].tolist(),
-        run_s={k: out[k]["run_s"] for k in ("cold_a", "cold_b", "warm")},
+        run_s={k: out[k]["run_s"] for k in ("cold_a", "cold_b", "warm", "mean")},
     )
     out["summary"] = summ
     return out

@@ -3,18 +3,24 @@
 A small MLP (5 inputs -> 4 x 256 GELU -> 4096 outputs; BF16 weights and
 activations, FP32 accumulation, trained and evaluated by the library's own
 MFMA kernels through opo_mlp_*) maps a sweep point's (pump power over
-threshold, cavity detuning, crystal length, GDD, mirror reflectivity) to a
-coarse steady-state spectrum: 1024 bins (free-spectral-range spacing, centred
-on the carrier) of signal and idler laid out as real/imaginary halves.  The
-trained content is the gauge-free amplitude spectrum (real halves; the
-imaginary halves are trained to zero), and warm fields take those amplitudes
-with random mode phases.  The solver starts from that prediction instead of
-noise and iterates to its own convergence test; the cold start stays the
-fallback and the judge of the prediction's error.
+threshold, cavity detuning, crystal length, GDD, mirror reflectivity) to its
+steady state in a gauge-free form (the signal and idler mode phases are free;
+only their sum is fixed by the pump):
+  - the amplitude shape: |spectrum| of signal and idler on 1024 bins
+    (free-spectral-range spacing, centred on the carrier), each normalised
+    to unit norm -- the real halves of the 2 x 1024 real/imag output;
+  - the power: log10(P / Ap0^2) / 10 of each field, replicated over that
+    field's imaginary half so the loss weights it like the shape.
+An off point (decayed below threshold) has a zero shape and the power floor.
+Warm fields take the predicted shape at the predicted power with random mode
+phases.  The solver starts from that instead of noise and iterates to its own
+convergence test; the cold start stays the fallback and the judge of the
+prediction's error.
 
 Data flow: `generate_dataset` runs batched sweeps (opo_batch_*) to steady
-state and encodes the fields; `Surrogate.fit` trains; `Surrogate.warm_fields`
-decodes predictions into time-domain fields on any grid N >= 1024.
+state and encodes the fields of the points that finished; `Surrogate.fit`
+trains; `Surrogate.warm_fields` decodes predictions into time-domain fields on
+any grid N >= 1024.
 """
 from __future__ import annotations
 
@@ -128,21 +134,41 @@ def gauge_fix(S, I):
     return S * ph, I * np.conj(ph)
 
 
+LOG_FLOOR = -20.0  # log10(P / Ap0^2) of an off point
+TARGET_KIND = "shape+logpower"
+
+
+def _log_power(P, scale):
+    return max(LOG_FLOOR, math.log10(P / scale ** 2)) if P > 0 else LOG_FLOOR
+
+
 def encode_target(As, Ai, scale):
-    """Training target: the gauge-free part of the steady state.  Signal and
-    idler mode phases are free (only their sum is fixed by the pump) and the
-    oscillating mode pair moves with detuning, so the learnable content is the
-    amplitude spectrum: |S| and |I| fill the real halves of the 2 x 1024
-    real/imag output, the imaginary halves are trained to zero."""
-    S, I = coarse_spectrum(As), coarse_spectrum(Ai)
-    z = np.zeros(S.size)
-    return (np.concatenate([np.abs(S), z, np.abs(I), z]) / scale).astype(np.float32)
+    """Training target of one steady state (As, Ai time-domain fields, scale =
+    the pump amplitude Ap0): [shape_s, logP_s, shape_i, logP_i], 1024 each."""
+    out = []
+    for A in (As, Ai):
+        a = np.abs(coarse_spectrum(A))
+        P = float(np.sum(a * a))
+        lp = _log_power(P, scale)
+        shape = a / math.sqrt(P) if lp > LOG_FLOOR else np.zeros_like(a)
+        out += [shape, np.full(a.size, lp / 10.0)]
+    return np.concatenate(out).astype(np.float32)
 
 
 def decode_target(y, scale):
-    y = np.asarray(y, np.float64) * scale
+    """Amplitude spectra (signal, idler) on the coarse grid from one output."""
+    y = np.asarray(y, np.float64)
     nb = y.size // 4
-    return y[:nb] + 1j * y[nb:2 * nb], y[2 * nb:3 * nb] + 1j * y[3 * nb:]
+    out = []
+    for k in range(2):
+        shape = np.maximum(y[2 * k * nb:(2 * k + 1) * nb], 0.0)
+        lp = 10.0 * float(np.mean(y[(2 * k + 1) * nb:(2 * k + 2) * nb]))
+        n = float(np.linalg.norm(shape))
+        if lp <= LOG_FLOOR + 1.0 or n == 0:
+            out.append(np.zeros(nb))
+            continue
+        out.append(shape / n * scale * 10.0 ** (lp / 2))
+    return out[0], out[1]
 
 
 # ------------------------------------------------------------ sweep points
@@ -178,7 +204,8 @@ def run_sweep(points, N=4096, NZ=25, max_rt=3000, check_every=50, tol=1e-6, prec
               warm=None, noise_sigma=1e-20, tiles=(0, 0)):
     """Run sweep points (rows of [nth_rel, delta, lcr_mm, gdd_fs2, R]) in one
     batch to convergence.  warm: optional list of (As, Ai) initial fields.
-    Returns (fields, rts, final power, device ms)."""
+    Returns (fields, rts, final power, device ms, derived scalars, states
+    (1 steady, 2 off, 0 capped, -1 diverged))."""
     made = []
     for k, pt in enumerate(points):
         a = sweep_args(*pt, N=N, NZ=NZ, NRT=max_rt)
@@ -194,47 +221,55 @@ def run_sweep(points, N=4096, NZ=25, max_rt=3000, check_every=50, tol=1e-6, prec
     for k, m in enumerate(made):
         b.upload(k, *m[2])
     try:
-        rts, _ = b.run_ex(max_rt, check_every=check_every, tol=tol, off_ratio=1e-6)
+        rts, states = b.run_ex(max_rt, check_every=check_every, tol=tol, off_ratio=1e-6)
     except OpoError as e:
         if e.status != -5:
             raise
-        rts = None
+        rts, states = None, None
     fields = [b.download(k) for k in range(len(points))]
     power = b.power()
     ms = b.last_run_ms()
     b.close()
-    return fields, rts, power, ms, [m[1] for m in made]
+    return fields, rts, power, ms, [m[1] for m in made], states
 
 
 def generate_dataset(n, seed=0, chunk=256, **kw):
-    """Steady states of n random sweep points: (features [n,5], targets
-    [n,4096], scales [n], rts [n])."""
+    """Encoded finished points (steady or off) of n random sweep points:
+    (features [m,5], targets [m,4096], scales [m], rts [m], states [m])."""
     pts = sample_points(n, seed)
-    X, Y, S, R = [], [], [], []
+    X, Y, S, R, ST = [], [], [], [], []
     for s in range(0, n, chunk):
         part = pts[s:s + chunk]
-        fields, rts, power, ms, ds = run_sweep(part, seed=seed * 100003 + s, **kw)
+        fields, rts, power, ms, ds, states = run_sweep(part, seed=seed * 100003 + s, **kw)
+        if states is None:
+            continue
         for k, pt in enumerate(part):
             _, As, Ai = fields[k]
-            if not (np.isfinite(As).all() and np.isfinite(Ai).all()):
+            if states[k] not in (1, 2) or not (np.isfinite(As).all() and np.isfinite(Ai).all()):
                 continue
             scale = ds[k].Ap0
             X.append(encode_features(*pt))
-            Y.append(encode_target(As, Ai, scale))
+            Y.append(encode_target(As, Ai, scale) if states[k] == 1 else
+                     encode_target(np.zeros_like(As), np.zeros_like(Ai), scale))
             S.append(scale)
-            R.append(-1 if rts is None else int(rts[k]))
-    return np.array(X), np.array(Y), np.array(S), np.array(R)
+            R.append(int(rts[k]))
+            ST.append(int(states[k]))
+    return np.array(X), np.array(Y), np.array(S), np.array(R), np.array(ST)
 
 
 class Surrogate:
-    """Trained MLP + the decoding onto solver grids."""
+    """Trained MLP + the decoding onto solver grids.  `mean_y` (the mean
+    training target) gives the input-independent seed used as the ablation
+    of the network."""
 
     def __init__(self, seed=0, device=0, max_batch=256):
         self.net = MLP(max_batch=max_batch, seed=seed, device=device)
+        self.mean_y = None
 
     def fit(self, X, Y, epochs=200, batch=256, lr=1e-3, seed=0, log=None):
         rng = np.random.default_rng(seed)
         n = X.shape[0]
+        self.mean_y = Y.mean(axis=0).astype(np.float32)
         hist = []
         for ep in range(epochs):
             order = rng.permutation(n)
@@ -250,24 +285,29 @@ class Surrogate:
     def predict(self, X):
         return self.net.forward(X)
 
-    def warm_fields_mixed(self, points, Ns, scales, floor_sigma=1e-12, seed=0):
+    def warm_fields_mixed(self, points, Ns, scales, floor_sigma=1e-12, seed=0, use_mean=False):
         """warm_fields for points on different grids (Ns[k] for point k)."""
         out = []
         for k, (pt, N) in enumerate(zip(points, Ns)):
-            out.append(self.warm_fields([pt], int(N), [scales[k]], floor_sigma, seed + 7919 * k)[0])
+            out.append(self.warm_fields([pt], int(N), [scales[k]], floor_sigma, seed + 7919 * k, use_mean)[0])
         return out
 
-    def warm_fields(self, points, N, scales, floor_sigma=1e-12, seed=0):
-        """Predicted (As, Ai) time-domain fields for sweep points on grid N."""
-        Y = self.predict(np.array([encode_features(*p) for p in points]))
+    def warm_fields(self, points, N, scales, floor_sigma=1e-12, seed=0, use_mean=False):
+        """Predicted (As, Ai) time-domain fields for sweep points on grid N
+        (use_mean: the mean training target for every point instead)."""
+        if use_mean:
+            if self.mean_y is None:
+                raise ValueError("no mean target: fit or load a trained surrogate first")
+            Y = np.tile(self.mean_y, (len(points), 1))
+        else:
+            Y = self.predict(np.array([encode_features(*p) for p in points]))
         rng = np.random.default_rng(seed)
         out = []
         for k in range(len(points)):
-            S, I = decode_target(Y[k], scales[k])
-            # predicted amplitudes (clipped at zero), free mode phases drawn at random
-            aS, aI = np.maximum(np.abs(S), 0), np.maximum(np.abs(I), 0)
-            S = aS * np.exp(2j * np.pi * rng.random(S.size))
-            I = aI * np.exp(2j * np.pi * rng.random(I.size))
+            aS, aI = decode_target(Y[k], scales[k])
+            # predicted amplitudes, free mode phases drawn at random
+            S = aS * np.exp(2j * np.pi * rng.random(aS.size))
+            I = aI * np.exp(2j * np.pi * rng.random(aI.size))
             As = field_from_coarse(S, N) + noise(seed + k, 0, floor_sigma, N)
             Ai = field_from_coarse(I, N) + noise(seed + k, 1, floor_sigma, N)
             out.append((As, Ai))

@@ -55,9 +55,18 @@ def test_spectrum_encoding_round_trip():
     k = np.argmax(np.abs(c))
     assert abs(s2[k].imag) < 1e-12 and s2[k].real > 0
     assert np.allclose(s2 * i2, c * I)
-    y = S.encode_target(A, A, 2.0)
+    y = S.encode_target(A, 0.5 * A, 2.0)
     s3, i3 = S.decode_target(y, 2.0)
-    assert np.allclose(s3.real, np.abs(c), atol=1e-6) and np.all(s3.imag == 0)
+    assert np.allclose(s3, np.abs(c), rtol=1e-5, atol=1e-6 * np.abs(c).max())
+    assert np.allclose(i3, 0.5 * np.abs(c), rtol=1e-5, atol=1e-6 * np.abs(c).max())
+    # shape halves have unit norm, power halves are constant
+    nb = 1024
+    assert abs(np.linalg.norm(y[:nb]) - 1) < 1e-5 and np.ptp(y[nb:2 * nb]) == 0
+    # an off point: zero shape, power floor, decodes to zero fields
+    z = S.encode_target(np.zeros(N, complex), np.zeros(N, complex), 2.0)
+    assert np.all(z[:nb] == 0) and np.allclose(z[nb:2 * nb], S.LOG_FLOOR / 10)
+    s4, i4 = S.decode_target(z, 2.0)
+    assert not s4.any() and not i4.any()
 
 
 def test_feature_encoding_range():
