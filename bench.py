@@ -43,6 +43,26 @@ def workload_c5(N=1 << 18, NZ=200, NRT=40):
                   Nth=4.0, R_pct=98.0, gdd_pct=100.0, upm=1, md=0.8 / np.pi, fm=4.0, lcr_mm=5.0)
 
 
+TUNING = None  # (col_threads, row_threads, xcd_pack, phase_mode) instead of autotuning
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def tune(obj):
+    """Autotune a Solver or Batch, or apply the fixed --tuning settings."""
+    if TUNING is None:
+        return obj.autotune(iters=10)
+    c, r, x, ph = TUNING
+    if hasattr(obj, "count"):
+        for t in obj.get_tuning():
+            obj.set_tuning(t["N"], col_threads=c, row_threads=r, xcd_pack=x, phase_mode=ph)
+        return obj.get_tuning()
+    obj.set_tuning(col_threads=c, row_threads=r, xcd_pack=x, phase_mode=ph)
+    return obj.get_tuning()
+
+
 def b_rt(N, NZ, nf, n_res, sc):
     """Algorithmic bytes of one round trip: NZ slices reading and writing every
     field once, plus the boundary (resonant fields read+write, pump read)."""
@@ -58,7 +78,9 @@ def c5_section(steps=20):
     d = inst["derived"]
     s = P.Solver(inst["params"])
     s.upload(inst["w"], inst["T"], inst["Ap_in"], inst["Ap"], inst["As"], inst["Ai"])
-    tuning = s.autotune(iters=10)
+    log("c5: tuning")
+    tuning = tune(s)
+    log("c5: running")
     s.run(0, 3)
     t0 = time.perf_counter()
     s.run(3, steps)
@@ -80,7 +102,9 @@ def c3_section(rts=10):
     from cuopo_amd import sweep as SW
     pts = SW.sweep_points(256, seed=3)
     b, made = SW.make_batch(pts, max_rt=rts + 10, precision=P.FP64)
-    tuning = b.autotune(iters=5)
+    log("c3: tuning")
+    tuning = tune(b)
+    log("c3: running")
     b.run(2, check_every=0, tol=0.0)
     t0 = time.perf_counter()
     b.run(rts, check_every=0, tol=0.0)
@@ -207,6 +231,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-shapes", action="store_true", help="skip the N=2^18 and batched FP64 sections")
+    ap.add_argument("--tuning", default=None,
+                    help="col_threads,row_threads,xcd_pack,phase_mode applied instead of autotuning "
+                         "(profiling runs: the profiler perturbs the autotuner's timings)")
+    ap.add_argument("--only", choices=["c5", "c3"], default=None,
+                    help="run only the N=2^18 (c5) or the batched FP64 (c3) section and print its JSON "
+                         "(for profiling those shapes)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "cuopo_amd", "data", "surrogate.npz"))
     a = ap.parse_args()
     if a.gpus != 1:
@@ -218,6 +248,15 @@ def main():
     import numpy as np
     import cuopo_amd as P
 
+    global TUNING
+    if a.tuning:
+        TUNING = tuple(int(v) for v in a.tuning.split(","))
+        if len(TUNING) != 4:
+            raise SystemExit("--tuning wants col_threads,row_threads,xcd_pack,phase_mode")
+    if a.only:
+        print(json.dumps(c5_section() if a.only == "c5" else c3_section()))
+        return
+
     args = workload_c2(NRT=a.warmup + a.steps + 1)
     inst = P.setup_instance(args, P.FP64, seed=1, noise_sigma=1e-20)
     d, prm = inst["derived"], inst["params"]
@@ -225,7 +264,9 @@ def main():
     s.upload(inst["w"], inst["T"], inst["Ap_in"], inst["Ap"], inst["As"], inst["Ai"])
     # launch settings chosen by timing the candidates on this instance (the
     # fields are restored afterwards; setup cost, outside the timed region)
-    tuning = s.autotune(iters=20)
+    log("c2: tuning")
+    tuning = s.autotune(iters=20) if TUNING is None else tune(s)
+    log("c2: running")
     # warm-up (also builds the round-trip graph used by the timed run)
     s.run(0, max(a.warmup, 3))
     # timed region: opo_run launches the graph replays and returns after the
@@ -276,8 +317,10 @@ def main():
         out["extra"]["c5_single_fp64"] = c5_section()
         out["extra"]["c3_batched_fp64"] = c3_section()
     if not a.no_sweep:
+        log("sweep")
         out["extra"]["sweep"] = sweep_section(a.weights)
     if not a.no_cpu:
+        log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(args, a.cpu_seconds)
     print(json.dumps(out))
 

@@ -217,7 +217,8 @@ void pick_tiles(Engine &E, Group &g) {
   // default column tile: one 128-byte line of columns (8 FP64 / 16 FP32)
   const int colt = g.col_threads ? g.col_threads : E.col_threads;
   int ct = colt ? std::max(1, colt / T1) : (int)(128 / esize(g));
-  int rtn = g.row_threads ? g.row_threads : (E.row_threads ? E.row_threads : 128);
+  // default row tile: 64 threads (the autotuner's choice on gfx950 for almost every group)
+  int rtn = g.row_threads ? g.row_threads : (E.row_threads ? E.row_threads : 64);
   ct = std::min(floor_pow2(ct), g.N2);
   while (ct > 1 && g.NF * ct * T1 > 512) ct >>= 1;
   g.ct = ct;

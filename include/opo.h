@@ -149,7 +149,7 @@ int opo_download_saved(opo_solver *s, double *save_p, double *save_s, double *sa
 /* sum |As|^2 after each of the last run's round trips (n values) */
 int opo_power_history(opo_solver *s, double *out, int n);
 /* Selects kernel tiles: col_threads (per field) / row_threads in
- * {32,64,128,256,512}; 0 = default (128-byte column tiles, 128-thread rows).
+ * {32,64,128,256,512}; 0 = default (128-byte column tiles, 64-thread row tiles).
  * No reference counterpart (autotuner hook). */
 int opo_set_tiles(opo_solver *s, int col_threads, int row_threads);
 /* Launch settings of the instances of one grid size (no reference
