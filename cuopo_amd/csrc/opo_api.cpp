@@ -118,6 +118,7 @@ struct Engine {
   int last_rt0 = 0, last_nrt = 0;
   int col_threads = 0, row_threads = 0;
   int xcd_pack = 0;
+  bool solver = false;  // one instance that stays active: stage kernels take its pointers directly
   double off_ratio = 1e-6;
   bool inst_dirty = true;
   std::vector<std::pair<int, int>> where;  // batch index -> (group, slot)
@@ -336,6 +337,7 @@ int upload_instance(Engine &E, Group &g, int k, const double *w, const double *T
     if (!I.lin) RCHK(dalloc(g, &I.lin, lin.size() * es));
     RCHK(upload_c(E, g, I.lin, lin.data(), lin.size()));
     E.inst_dirty = true;
+    E.graph_key = -1;  // launch arguments may carry this instance's pointers
   }
 
   // GDD compensation, cuOPO.cu:521-553 (AddGDD functions.cuh:156-169; AddGDDTOD :173-186)
@@ -460,6 +462,12 @@ int emit_col(Engine &E, Group &g, unsigned ops, int load, int inv, int fwd, int 
   a.snap_every = snap_every;
   a.xcd_pack = g.xcd_pack >= 0 ? g.xcd_pack : E.xcd_pack;
   a.swz = !a.xcd_pack && g.ct * esize(g) < 128 && (g.N2 / g.ct) % 8 == 0;
+  if (E.solver) {
+    const Instance &I = g.inst[0];
+    a.direct = 1;
+    a.d_state = I.state; a.d_nl = I.nl;
+    a.d_kp = I.prm.kp; a.d_ks = I.prm.ks; a.d_h = I.prm.dz / 2;
+  }
   const int threads_ct = g.ct;
   int e = opo::launch_col(g.fp32, g.N1, g.NF, threads_ct, (int)g.members.size(), g.d_insts, a, g.cur, e0, e1);
   if (e != 0) return fail(OPO_ERR_HIP, std::string("col_stage launch: ") + hipGetErrorString((hipError_t)e));
@@ -476,6 +484,12 @@ int emit_row(Engine &E, Group &g, bool gdd, int mask, hipEvent_t e0 = nullptr, h
   a.nf = nf; a.f0 = fl[0]; a.f1 = fl[1]; a.f2 = fl[2];
   a.xcd_pack = g.xcd_pack >= 0 ? g.xcd_pack : E.xcd_pack;
   a.xl_bits = g.xl_bits;
+  if (E.solver) {
+    const Instance &I = g.inst[0];
+    a.direct = 1;
+    a.d_state = I.state; a.d_lin = I.lin; a.d_gdd = I.gdd;
+    a.d_lfac = I.use_fac && group_factored(g) ? I.lfac : nullptr;
+  }
   int e = opo::launch_row(g.fp32, g.N2, g.NF, g.rt, (int)g.members.size(), 0, g.d_insts, a, g.cur, e0, e1);
   if (e != 0) return fail(OPO_ERR_HIP, std::string("row_stage launch: ") + hipGetErrorString((hipError_t)e));
   return OPO_OK;
@@ -738,6 +752,7 @@ int opo_create(const opo_params *p, opo_solver **out) {
   *out = nullptr;
   std::unique_ptr<opo_solver> s(new opo_solver());
   RCHK(engine_init(s->E, p, 1));
+  s->E.solver = true;
   *out = s.release();
   return OPO_OK;
 }

@@ -77,6 +77,12 @@ struct ColArgs {
   int snap_every;    // OP_SNAP fires when (rt + 1) % snap_every == 0
   int xcd_pack;      // 1: grid is 8x wider and only blocks with blockIdx.x % 8 == 0 work
   int swz;           // 1: XCD-contiguous tile order (tiles narrower than a cache line)
+  // single always-active instance (solver): the slice-interior kernel takes
+  // these instead of reading the instance descriptor first
+  int direct;
+  void *d_state;
+  const double *d_nl;
+  double d_kp, d_ks, d_h;
 };
 
 struct RowArgs {
@@ -86,6 +92,9 @@ struct RowArgs {
   int use_gdd;
   int xcd_pack;      // as in ColArgs
   int xl_bits;       // L of the cross-term tables of OpoInst::lfac
+  int direct;        // as in ColArgs: pointers below instead of the descriptor
+  void *d_state;
+  const void *d_lin, *d_lfac, *d_gdd;
 };
 
 struct TickArgs {

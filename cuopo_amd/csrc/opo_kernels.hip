@@ -50,13 +50,14 @@ template <int NF, typename V> struct NlCoef {
   decltype(V::x) kp, ks, h;
 };
 
-template <int NF, typename V> __device__ __forceinline__ NlCoef<NF, V> load_nl(const OpoInst *I, int s) {
+template <int NF, typename V>
+__device__ __forceinline__ NlCoef<NF, V> load_nl(const double *nl, double kp, double ks, double h, int s) {
   NlCoef<NF, V> r;
-  const double *q = I->nl + 16 * s;
+  const double *q = nl + 16 * s;
 #pragma unroll
   for (int j = 0; j < (NF == 3 ? 3 : 6); ++j) r.c[j] = ld_c<V>(q + 2 * j);
   using R = decltype(V::x);
-  r.kp = (R)I->kp; r.ks = (R)I->ks; r.h = (R)I->h;
+  r.kp = (R)kp; r.ks = (R)ks; r.h = (R)h;
   return r;
 }
 
@@ -108,8 +109,9 @@ __device__ __forceinline__ void nl_half_pt(V *xp, V *xs, V *xi, const NlCoef<NF,
 // for the pointwise program, which every group evaluates for its 8 samples;
 // each group keeps its own field.  MID compiles the slice-interior variant
 // only (inverse -> NL end -> NL start -> forward).
-template <typename T, int N1, int NF, bool MID>
+template <typename T, int N1, int NF, bool MID, bool DIRECT>
 __global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ insts, ColArgs a) {
+  static_assert(MID || !DIRECT, "direct arguments serve the slice-interior kernel");
   using V = typename vec2<T>::type;
   constexpr int T1 = N1 / 8;
   extern __shared__ unsigned char smem_raw[];
@@ -117,7 +119,9 @@ __global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ ins
   V *lds = ltw + N1;                         // 2 buffers x NF fields x CT*N1
 
   const OpoInst *I = insts + blockIdx.y;
-  if (I->active_v == 0) return;  // block-uniform
+  if constexpr (!DIRECT) {
+    if (I->active_v == 0) return;  // block-uniform
+  }
   // XCD packing: work-groups are dispatched round-robin over the 8 XCDs, so
   // keeping only every 8th block puts the whole stage on one XCD (and its L2)
   if (a.xcd_pack && (blockIdx.x & 7)) return;
@@ -138,7 +142,7 @@ __global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ ins
   const int n2 = bx * CT + c;
   const long long N = (long long)N1 * N2;
   const unsigned nmask = (unsigned)(N - 1);
-  V *st = reinterpret_cast<V *>(I->state);
+  V *st = reinterpret_cast<V *>(DIRECT ? a.d_state : I->state);
   const int fstride = CT * N1;
   const int bufstride = NF * fstride;
   const int inv_mask = MID ? (1 << NF) - 1 : a.inv_mask;
@@ -149,8 +153,13 @@ __global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ ins
 
   const unsigned ops = MID ? (OP_NL_END | OP_NL_START) : a.ops;
   NlCoef<NF, V> nle{}, nls{};
-  if (ops & OP_NL_END) nle = load_nl<NF, V>(I, a.s_end);
-  if (ops & OP_NL_START) nls = load_nl<NF, V>(I, a.s_start);
+  if constexpr (DIRECT) {
+    nle = load_nl<NF, V>(a.d_nl, a.d_kp, a.d_ks, a.d_h, a.s_end);
+    nls = load_nl<NF, V>(a.d_nl, a.d_kp, a.d_ks, a.d_h, a.s_start);
+  } else {
+    if (ops & OP_NL_END) nle = load_nl<NF, V>(I->nl, I->kp, I->ks, I->h, a.s_end);
+    if (ops & OP_NL_START) nls = load_nl<NF, V>(I->nl, I->kp, I->ks, I->h, a.s_start);
+  }
   if (any_fft) {
     const V *tw1 = reinterpret_cast<const V *>(a.tw1);
     for (int e = tid; e < N1; e += blockDim.x) ltw[e] = tw1[e];
@@ -325,7 +334,7 @@ __global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ ins
   }
 }
 
-template <typename T, int N2, int NF>
+template <typename T, int N2, int NF, bool DIRECT>
 __global__ __launch_bounds__(512) void row_stage(const OpoInst *__restrict__ insts, RowArgs a) {
   using V = typename vec2<T>::type;
   constexpr int T2 = N2 / 8;
@@ -334,7 +343,9 @@ __global__ __launch_bounds__(512) void row_stage(const OpoInst *__restrict__ ins
   V *lds = reinterpret_cast<V *>(smem_raw);  // 2 buffers x RT padded rows
 
   const OpoInst *I = insts + blockIdx.y;
-  if (I->active_v == 0) return;
+  if constexpr (!DIRECT) {
+    if (I->active_v == 0) return;
+  }
   if (a.xcd_pack && (blockIdx.x & 7)) return;  // see col_stage
   const int bx = a.xcd_pack ? (int)(blockIdx.x >> 3) : (int)blockIdx.x;
 
@@ -347,13 +358,13 @@ __global__ __launch_bounds__(512) void row_stage(const OpoInst *__restrict__ ins
   const int r = tid / T2;
   const int k1 = (bx - fi * tiles_per_field) * RT + r;
   const long long N = (long long)a.N1 * N2;
-  V *row = reinterpret_cast<V *>(I->state) + f * N + (long long)N2 * k1;
+  V *row = reinterpret_cast<V *>(DIRECT ? a.d_state : I->state) + f * N + (long long)N2 * k1;
 
   V v[8], ph_tab[8], treg[tw_count(N2) > 0 ? tw_count(N2) : 1];
-  const V *fac = reinterpret_cast<const V *>(I->lfac);
+  const V *fac = reinterpret_cast<const V *>(DIRECT ? a.d_lfac : I->lfac);
   const bool use_fac = !a.use_gdd && fac;
-  const V *tab = a.use_gdd ? reinterpret_cast<const V *>(I->gdd) + (f - 1) * N
-                          : reinterpret_cast<const V *>(I->lin) + f * N;
+  const V *tab = a.use_gdd ? reinterpret_cast<const V *>(DIRECT ? a.d_gdd : I->gdd) + (f - 1) * N
+                          : reinterpret_cast<const V *>(DIRECT ? a.d_lin : I->lin) + f * N;
   // phase loads first (the factorised products only wait for these), then
   // the row, then the twiddles
   V xr = mk<V>(0, 0);
@@ -447,12 +458,15 @@ static int launch_col_t(int ct, int count, const OpoInst *insts, const ColArgs &
   const bool mid = a.ops == (OP_NL_END | OP_NL_START) && a.inv_mask == (1 << NF) - 1 &&
                    a.fwd_mask == (1 << NF) - 1 && a.load_mask == (1 << NF) - 1 && a.store_mask == (1 << NF) - 1;
   const size_t lds = ((size_t)N1 + 2 * (size_t)NF * ct * N1) * sizeof(V) + 64 * sizeof(double);
-  static bool big_lds[2] = {false, false};  // gfx950 has 160 KiB of LDS; opt in above 64 KiB once
-  auto kern = mid ? &col_stage<T, N1, NF, true> : &col_stage<T, N1, NF, false>;
-  if (lds > 65536 && !big_lds[mid]) {
+  const bool direct = mid && a.direct;
+  static bool big_lds[3] = {false, false, false};  // gfx950 has 160 KiB of LDS; opt in above 64 KiB once
+  const int kv = direct ? 2 : (mid ? 1 : 0);
+  auto kern = direct ? &col_stage<T, N1, NF, true, true>
+                     : (mid ? &col_stage<T, N1, NF, true, false> : &col_stage<T, N1, NF, false, false>);
+  if (lds > 65536 && !big_lds[kv]) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                               160 * 1024);
-    big_lds[mid] = true;
+    big_lds[kv] = true;
   }
   if (e0 || e1) hipExtLaunchKernelGGL(kern, grid, dim3(threads), lds, s, e0, e1, 0, insts, a);
   else hipLaunchKernelGGL(kern, grid, dim3(threads), lds, s, insts, a);
@@ -467,14 +481,15 @@ static int launch_row_t(int rt, int count, int tiles_y, const OpoInst *insts, co
   const size_t lds = 2 * (size_t)rt * (N2 + N2 / 8) * sizeof(V);
   dim3 grid((a.N1 / rt) * a.nf * (a.xcd_pack ? 8 : 1), count);
   (void)tiles_y;
-  static bool big_lds = false;
-  if (lds > 65536 && !big_lds) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&row_stage<T, N2, NF>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    big_lds = true;
+  static bool big_lds[2] = {false, false};
+  auto kern = a.direct ? &row_stage<T, N2, NF, true> : &row_stage<T, N2, NF, false>;
+  if (lds > 65536 && !big_lds[a.direct ? 1 : 0]) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    big_lds[a.direct ? 1 : 0] = true;
   }
-  if (e0 || e1) hipExtLaunchKernelGGL((row_stage<T, N2, NF>), grid, dim3(threads), lds, s, e0, e1, 0, insts, a);
-  else hipLaunchKernelGGL((row_stage<T, N2, NF>), grid, dim3(threads), lds, s, insts, a);
+  if (e0 || e1) hipExtLaunchKernelGGL(kern, grid, dim3(threads), lds, s, e0, e1, 0, insts, a);
+  else hipLaunchKernelGGL(kern, grid, dim3(threads), lds, s, insts, a);
   return (int)hipGetLastError();
 }
 
