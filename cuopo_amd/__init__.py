@@ -115,6 +115,8 @@ SIGNATURES = {
     "opo_get_tuning": (_i, [_p, _p]),
     "opo_set_tuning": (_i, [_p, _p]),
     "opo_batch_autotune": (_i, [_p, _i]),
+    "opo_batch_enable_save": (_i, [_p, _i]),
+    "opo_batch_download_saved": (_i, [_p, _i, _p, _p, _p]),
     "opo_batch_get_tuning": (_i, [_p, _p, _i]),
     "opo_batch_set_tuning": (_i, [_p, _p]),
     "opo_batch_time_stages": (_i, [_p, _i, _i, C.POINTER(_d), C.POINTER(_d)]),

@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "../../include/opo.h"
+#include "opo_cli_common.h"
 
 #if !defined(CW_OPO) && !defined(NS_OPO)
 #error "define the regime: -DCW_OPO or -DNS_OPO (src/cuOPO.cu:65-70)"
@@ -64,23 +65,6 @@ double seconds() {
   return duration<double>(steady_clock::now().time_since_epoch()).count();
 }
 
-// One value per line at 20 significant digits (SaveVectorReal/Complex,
-// src/headers/files.cuh:21-62); complex arrays split into _r/_i files.
-bool save_real(const std::string &path, const double *v, size_t n) {
-  FILE *f = std::fopen(path.c_str(), "w");
-  if (!f) return false;
-  std::vector<char> buf(1 << 20);
-  std::setvbuf(f, buf.data(), _IOFBF, buf.size());
-  for (size_t i = 0; i < n; ++i) std::fprintf(f, "%.20g\n", v[i]);
-  std::fclose(f);
-  return true;
-}
-bool save_complex(const std::string &stem, const double *v, size_t n) {
-  std::vector<double> re(n), im(n);
-  for (size_t i = 0; i < n; ++i) { re[i] = v[2 * i]; im[i] = v[2 * i + 1]; }
-  return save_real(stem + "_r.dat", re.data(), n) && save_real(stem + "_i.dat", im.data(), n);
-}
-
 void usage(const char *prog) {
   std::fprintf(stderr,
                "usage: %s <LP_nm> <T_C> <Lambda_um> <N> <R_%%> <delta_pi> <GDD_%%> <TOD_%%> <UPM> <beta_pi> <df> "
@@ -92,36 +76,26 @@ void usage(const char *prog) {
 }  // namespace
 
 int main(int argc, char *argv[]) {
-  std::cout << "\n\n\n#######---Welcome to OPO simulator---#######\n\n\n" << std::endl;
-  time_t current_time;
-  time(&current_time);
-  std::cout << "Date: " << ctime(&current_time) << std::endl;
-  const double iStart = seconds();
-#ifdef CW_OPO
-  std::cout << "Regime: continuous wave" << std::endl;
-#endif
 #ifdef NS_OPO
-  std::cout << "Regime: nanosecond" << std::endl;
+  const int regime = OPO_NS;
+#else
+  const int regime = OPO_CW;
 #endif
 #ifdef THREE_EQS
-  std::cout << "Three equations" << std::endl;
+  const int three = 1;
 #else
-  std::cout << "Two equations" << std::endl;
+  const int three = 0;
 #endif
+  opo_cli::print_banner(std::cout, regime, three);
+  const double iStart = seconds();
   if (argc < 13) {
     usage(argv[0]);
     return 1;
   }
 
   opo_args a{};
-#ifdef THREE_EQS
-  a.three_eqs = 1;
-#endif
-#ifdef NS_OPO
-  a.regime = OPO_NS;
-#else
-  a.regime = OPO_CW;
-#endif
+  a.three_eqs = three;
+  a.regime = regime;
 #ifdef SPPLT
   a.crystal = OPO_SPPLT;
 #else
@@ -169,75 +143,13 @@ int main(int argc, char *argv[]) {
   if (d.three_eqs) opo_noise(seed, 1, sigma, Ai.data(), N);
 
   if (save_inputs) {  // src/cuOPO.cu:272-286
-    save_complex(out + "/pump_input", Ap_in.data(), n_ap);
-    save_complex(out + "/signal_input", As.data(), N);
-    if (d.three_eqs) save_complex(out + "/idler_input", Ai.data(), N);
+    opo_cli::write_complex(out + "/pump_input", Ap_in.data(), n_ap);
+    opo_cli::write_complex(out + "/signal_input", As.data(), N);
+    if (d.three_eqs) opo_cli::write_complex(out + "/idler_input", Ai.data(), N);
   }
 
   // parameter dump, src/cuOPO.cu:289-369
-  std::cout << "\n\nSimulation parameters:\n\n " << std::endl;
-  std::cout << "Number of round trips   = " << d.NRT << std::endl;
-  std::cout << "Pump wavelength         = " << d.lp * 1e3 << " nm" << std::endl;
-  std::cout << "Signal wavelength       = " << d.ls * 1e3 << " nm" << std::endl;
-  if (d.three_eqs) std::cout << "Idler wavelength        = " << d.li * 1e3 << " nm" << std::endl;
-  std::cout << "Temp                    = " << d.Temp << " ºC" << std::endl;
-  std::cout << "np                      = " << d.np << std::endl;
-  std::cout << "ns                      = " << d.ns << std::endl;
-  if (d.three_eqs) std::cout << "ni                      = " << d.ni << std::endl;
-  std::cout << "ν⁻¹ pump                = " << 1.0 / d.vp << " ps/μm" << std::endl;
-  std::cout << "ν⁻¹ signal              = " << 1.0 / d.vs << " ps/μm" << std::endl;
-  if (d.three_eqs) std::cout << "ν⁻¹ idler               = " << 1.0 / d.vi << " ps/μm" << std::endl;
-  std::cout << "Δk                      = " << d.dk << " μm⁻¹" << std::endl;
-  std::cout << "Δk'                     = " << d.dkp << " ps/μm" << std::endl;
-  std::cout << "GVD pump                = " << d.b2p << " ps²/μm" << std::endl;
-  std::cout << "GVD signal              = " << d.b2s << " ps²/μm" << std::endl;
-  if (d.three_eqs) std::cout << "GVD idler               = " << d.b2i << " ps²/μm" << std::endl;
-  std::cout << "GVD compensation        = " << std::atoi(argv[7]) << " %" << std::endl;
-  std::cout << "Net GVD                 = " << (1 - d.epsilon) * d.b2s << " ps²/μm" << std::endl;
-  std::cout << "TOD pump                = " << d.b3p << " ps³/μm" << std::endl;
-  std::cout << "TOD signal              = " << d.b3s << " ps³/μm" << std::endl;
-  std::cout << "TOD compensation        = " << std::atof(argv[8]) << " %" << std::endl;
-  std::cout << "Net TOD                 = " << (1 - 0.01 * std::atoi(argv[8])) * d.b3s << " ps³/μm" << std::endl;
-  std::cout << "dQ                      = " << d.dQ * 1e6 << " pm/V" << std::endl;
-  std::cout << "Λ                       = " << d.Lambda << " μm" << std::endl;
-  std::cout << "αcp                     = " << d.alpha_crp << " μm⁻¹" << std::endl;
-  std::cout << "αcs                     = " << d.alpha_crs << " μm⁻¹" << std::endl;
-  std::cout << "αs                      = " << d.alphas << std::endl;
-  if (d.three_eqs) {
-    std::cout << "αci                      = " << d.alpha_cri << " μm⁻¹" << std::endl;
-    std::cout << "αi                      = " << d.alphai << std::endl;
-  }
-  std::cout << "Crystal length          = " << d.Lcr * 1e-3 << " mm" << std::endl;
-  std::cout << "Cavity  length          = " << d.Lcav * 1e-3 << " mm" << std::endl;
-  std::cout << "Δz                      = " << d.dz << " μm" << std::endl;
-  std::cout << "Reflectivity (signal)   = " << d.Rs * 100 << " %" << std::endl;
-  if (d.three_eqs) std::cout << "Reflectivity (idler)    = " << d.Ri * 100 << " %" << std::endl;
-  std::cout << "Cavity Finesse          = " << d.finesse << std::endl;
-  std::cout << "Cavity lw (FWHM)        = " << d.lw << " MHz" << std::endl;
-  std::cout.precision(15);
-  std::cout << "Round-trip time         = " << d.t_rt << " ps" << std::endl;
-  std::cout << "FSR                     = " << d.FSR * 1e3 << " GHz" << std::endl;
-  std::cout << "Cavity detuning (δ)     = " << d.delta << "π" << std::endl;
-  std::cout << "Using N                 = " << d.N << " points" << std::endl;
-  std::cout << "dT                      = " << d.dT << " ps" << std::endl;
-  std::cout << "SIZEL                   = " << d.SIZEL << std::endl;
-  std::cout << "dFp                     = " << d.dFp << " THz" << std::endl;
-  std::cout << "Max frequency           = " << Fp[d.SIZEL - 1] << " THz" << std::endl;
-  std::cout << "Ap0                     = " << d.Ap0 << " V/um" << std::endl;
-  std::cout << "waist                   = " << d.waist << " μm" << std::endl;
-  std::cout << "spot                    = " << d.spot << " μm²" << std::endl;
-  std::cout << "Power threshold         = " << d.Pth << " W" << std::endl;
-  std::cout << "Power                   = " << d.Power << " W" << std::endl;
-  std::cout << "Times above the thres.  = " << d.Nth << std::endl;
-  std::cout << "(ΓL)²                   = " << d.GammaL * d.GammaL << std::endl;
-  if (d.use_eom) {
-    std::cout << "Using a phase modulator:" << std::endl;
-    std::cout << "Mod. depth (β)          = " << a.md << "π rad = " << d.mod_depth << " rad" << std::endl;
-    std::cout << "Freq. detuning (δf)     = " << d.df * 1e6 << " MHz" << std::endl;
-    std::cout << "Mod. frequency(fm)      = " << d.fpm * 1e3 << " GHz" << std::endl;
-  } else {
-    std::cout << "No phase modulator" << std::endl;
-  }
+  opo_cli::print_params(std::cout, d, a, std::atof(argv[7]), std::atof(argv[8]), Fp[d.SIZEL - 1]);
 
   // solver (device allocation + uploads, src/cuOPO.cu:379-442)
   opo_params p{};
@@ -260,19 +172,20 @@ int main(int argc, char *argv[]) {
 
   // outputs, src/cuOPO.cu:620-635
   std::cout << "\nSaving time and frequency vectors...\n" << std::endl;
-  save_real(out + "/Tp.dat", Tp.data(), d.SIZEL);
-  save_real(out + "/freq.dat", Fp.data(), d.SIZEL);
-  save_real(out + "/T.dat", T.data(), N);
+  opo_cli::write_real(out + "/Tp.dat", Tp.data(), d.SIZEL);
+  opo_cli::write_real(out + "/freq.dat", Fp.data(), d.SIZEL);
+  opo_cli::write_real(out + "/T.dat", T.data(), N);
   std::vector<double> sp(2 * d.SIZEL), ss(2 * d.SIZEL), si(d.three_eqs ? 2 * d.SIZEL : 0);
   CHECK(opo_download_saved(s, sp.data(), ss.data(), d.three_eqs ? si.data() : nullptr));
-  save_complex(out + "/signal_output", ss.data(), d.SIZEL);
-  save_complex(out + "/pump_output", sp.data(), d.SIZEL);
-  if (d.three_eqs) save_complex(out + "/idler_output", si.data(), d.SIZEL);
+  opo_cli::write_complex(out + "/signal_output", ss.data(), d.SIZEL);
+  opo_cli::write_complex(out + "/pump_output", sp.data(), d.SIZEL);
+  if (d.three_eqs) opo_cli::write_complex(out + "/idler_output", si.data(), d.SIZEL);
   CHECK(opo_destroy(s));
 
   const double iElaps = seconds() - iStart;
   if (iElaps > 60) std::cout << "\n\n...time elapsed " << iElaps / 60.0 << " min\n\n " << std::endl;
   else std::cout << "\n\n...time elapsed " << iElaps << " seconds\n\n " << std::endl;
+  time_t current_time;
   time(&current_time);
   std::cout << ctime(&current_time) << std::endl;
   return 0;

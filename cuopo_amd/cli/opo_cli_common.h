@@ -1,0 +1,29 @@
+// opo_cli_common.h -- pieces shared by the cuOPO drop-in executable and the
+// batched sweep driver: the .dat writer (src/headers/files.cuh:21-75), the
+// banner and the parameter dump (src/cuOPO.cu:52-64,289-369).
+#pragma once
+
+#include <cstddef>
+#include <ostream>
+#include <string>
+
+#include "../../include/opo.h"
+
+namespace opo_cli {
+
+// One value per line at 20 significant digits (SaveVectorReal,
+// files.cuh:21-40).  Formatting runs on `threads` host threads (0 = all
+// hardware threads), the file is written in order.
+bool write_real(const std::string &path, const double *v, size_t n, int threads = 0);
+// Interleaved complex -> stem_r.dat / stem_i.dat (SaveVectorComplex,
+// files.cuh:43-62).
+bool write_complex(const std::string &stem, const double *v, size_t n, int threads = 0);
+
+// "Welcome", date, regime and equation lines of src/cuOPO.cu:52-64, 75-88.
+void print_banner(std::ostream &o, int regime, int three_eqs);
+// The parameter dump of src/cuOPO.cu:289-369.  gdd_arg / tod_arg are argv7 /
+// argv8 as given (the reference prints them back).
+void print_params(std::ostream &o, const opo_derived &d, const opo_args &a, double gdd_arg, double tod_arg,
+                  double max_freq);
+
+}  // namespace opo_cli

@@ -784,23 +784,46 @@ int opo_set_fields(opo_solver *s, const double *Ap, const double *As, const doub
   return upload_c(E, g, g.inst[0].state, st.data(), st.size());
 }
 
-int opo_enable_save(opo_solver *s, int enable) {
-  if (!s) return fail(OPO_ERR_INVALID, "null solver");
-  Engine &E = s->E;
-  Group &g = E.groups[0];
-  Instance &I = g.inst[0];
+namespace {
+
+// save slots (CW: the last Nrts round trips; NS: every round trip) for every
+// instance of the engine, src/cuOPO.cu:570-608
+int engine_enable_save(Engine &E, int enable) {
   if (enable) {
-    const size_t slots = I.prm.regime == OPO_CW ? (size_t)I.prm.Nrts : (size_t)I.prm.NRT;
-    for (int f = 0; f < g.NF; ++f)
-      if (!I.save[f]) {
-        RCHK(dalloc(g, &I.save[f], slots * g.N * esize(g)));
-        HIPCHK(hipMemsetAsync(I.save[f], 0, slots * g.N * esize(g), E.stream));
+    for (auto &g : E.groups)
+      for (auto &I : g.inst) {
+        const size_t slots = I.prm.regime == OPO_CW ? (size_t)I.prm.Nrts : (size_t)I.prm.NRT;
+        for (int f = 0; f < g.NF; ++f)
+          if (!I.save[f]) {
+            RCHK(dalloc(g, &I.save[f], slots * g.N * esize(g)));
+            HIPCHK(hipMemsetAsync(I.save[f], 0, slots * g.N * esize(g), E.stream));
+          }
       }
+    HIPCHK(hipStreamSynchronize(E.stream));
   }
   E.save_enabled = enable != 0;
   E.inst_dirty = true;
   E.graph_key = -1;
   return OPO_OK;
+}
+
+int engine_download_saved(Engine &E, Group &g, Instance &I, double *sp, double *ss, double *si) {
+  const size_t slots = I.prm.regime == OPO_CW ? (size_t)I.prm.Nrts : (size_t)I.prm.NRT;
+  double *dst[3] = {sp, ss, si};
+  for (int f = 0; f < g.NF; ++f) {
+    if (!dst[f]) continue;
+    if (!I.save[f]) return fail(OPO_ERR_STATE, "saving was not enabled");
+    RCHK(download_c(E, g, reinterpret_cast<cd *>(dst[f]), I.save[f], slots * g.N));
+  }
+  return OPO_OK;
+}
+
+}  // namespace
+
+int opo_enable_save(opo_solver *s, int enable) {
+  if (!s) return fail(OPO_ERR_INVALID, "null solver");
+  HIPCHK(hipSetDevice(s->E.device));
+  return engine_enable_save(s->E, enable);
 }
 
 int opo_set_tiles(opo_solver *s, int col_threads, int row_threads) {
@@ -916,17 +939,8 @@ int opo_download(opo_solver *s, double *Ap, double *As, double *Ai) {
 
 int opo_download_saved(opo_solver *s, double *sp, double *ss, double *si) {
   if (!s) return fail(OPO_ERR_INVALID, "null solver");
-  Engine &E = s->E;
-  Group &g = E.groups[0];
-  Instance &I = g.inst[0];
-  const size_t slots = I.prm.regime == OPO_CW ? (size_t)I.prm.Nrts : (size_t)I.prm.NRT;
-  double *dst[3] = {sp, ss, si};
-  for (int f = 0; f < g.NF; ++f) {
-    if (!dst[f]) continue;
-    if (!I.save[f]) return fail(OPO_ERR_STATE, "saving was not enabled");
-    RCHK(download_c(E, g, reinterpret_cast<cd *>(dst[f]), I.save[f], slots * g.N));
-  }
-  return OPO_OK;
+  HIPCHK(hipSetDevice(s->E.device));
+  return engine_download_saved(s->E, s->E.groups[0], s->E.groups[0].inst[0], sp, ss, si);
 }
 
 int opo_power_history(opo_solver *s, double *out, int n) {
@@ -1031,6 +1045,20 @@ int opo_batch_create(const opo_params *params, int count, opo_batch **out) {
   RCHK(engine_init(b->E, params, count));
   *out = b.release();
   return OPO_OK;
+}
+
+int opo_batch_enable_save(opo_batch *b, int enable) {
+  if (!b) return fail(OPO_ERR_INVALID, "null batch");
+  HIPCHK(hipSetDevice(b->E.device));
+  return engine_enable_save(b->E, enable);
+}
+
+int opo_batch_download_saved(opo_batch *b, int idx, double *sp, double *ss, double *si) {
+  if (!b || idx < 0 || idx >= b->E.count) return fail(OPO_ERR_INVALID, "bad batch index");
+  HIPCHK(hipSetDevice(b->E.device));
+  const auto gs = b->E.where[idx];
+  Group &g = b->E.groups[gs.first];
+  return engine_download_saved(b->E, g, g.inst[gs.second], sp, ss, si);
 }
 
 int opo_batch_destroy(opo_batch *b) {

@@ -225,6 +225,11 @@ int opo_batch_set_tuning(opo_batch *b, const opo_tuning *t);
 /* opo_time_stages for the group of grid size N. */
 int opo_batch_time_stages(opo_batch *b, int N, int iters, double *col_us, double *row_us);
 int opo_batch_set_tiles(opo_batch *b, int col_threads, int row_threads);
+/* Save slots for every instance (opo_enable_save per instance; the round
+ * trips of src/cuOPO.cu:570-608) and their download (slots*N complex each,
+ * any pointer may be NULL). */
+int opo_batch_enable_save(opo_batch *b, int enable);
+int opo_batch_download_saved(opo_batch *b, int idx, double *save_p, double *save_s, double *save_i);
 int opo_batch_destroy(opo_batch *b);
 
 /* ---------------- warm-start surrogate (no reference counterpart) ----------
