@@ -1,0 +1,103 @@
+"""Edge shapes of the round trip against the FP64 oracle (GPU, C ABI): one or
+two slices per pass (no slice-interior kernel / one), the smallest grid
+(N = 256, 16 x 16 four-step split), the largest (N = 2^18, the NS workload
+with EOM and GDD at full size), FP32C at the largest grid, and a linear step
+whose only effect is absorption (a size-independent closed form)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.helpers import c1, c2, c5, inputs, rel_l2, to_args
+
+pytestmark = pytest.mark.gpu
+
+
+def solver(P, cfg, precision=0):
+    return P.Solver(P.params_from(P.derive(to_args(cfg)), precision))
+
+
+def grown(As, Ai, f=1e19):
+    return As * f, (Ai * f if Ai is not None else None)
+
+
+@pytest.mark.parametrize("NZ", [1, 2])
+@pytest.mark.parametrize("three", [False, True])
+def test_one_and_two_slices(gpu, NZ, three):
+    cfg = c2(N=2048, NZ=NZ, NRT=10) if three else c1(N=2048, NZ=NZ, NRT=10)
+    p, T, w, Ap_in, Ap, As, Ai = inputs(cfg, seed=9)
+    As, Ai = grown(As, Ai)
+    s = solver(gpu, cfg)
+    s.upload(w, T, Ap_in, Ap, As, Ai)
+    s.run(0, 5)  # graph replay path (more than two round trips)
+    gp, gs, gi = s.download()
+    O.run(p, w, T, Ap_in, Ap, As, Ai, 0, 5)
+    assert rel_l2(gp, Ap) < 1e-9 and rel_l2(gs, As) < 1e-9
+    if three:
+        assert rel_l2(gi, Ai) < 1e-9
+
+
+@pytest.mark.parametrize("three", [False, True])
+def test_smallest_grid(gpu, three):
+    cfg = c2(N=256, NZ=8, NRT=10) if three else c1(N=256, NZ=8, NRT=10)
+    p, T, w, Ap_in, Ap, As, Ai = inputs(cfg, seed=4)
+    As, Ai = grown(As, Ai)
+    s = solver(gpu, cfg)
+    s.upload(w, T, Ap_in, Ap, As, Ai)
+    s.run(0, 3)
+    gp, gs, gi = s.download()
+    O.run(p, w, T, Ap_in, Ap, As, Ai, 0, 3)
+    assert rel_l2(gp, Ap) < 1e-9 and rel_l2(gs, As) < 1e-9
+    if three:
+        assert rel_l2(gi, Ai) < 1e-9
+
+
+def test_largest_grid_ns_eom_gdd(gpu):
+    """The N = 2^18 nanosecond workload (EOM and GDD on), 4 slices, 2 round
+    trips: second round trip reads pump slot 1."""
+    cfg = c5(N=1 << 18, NZ=4, NRT=3)
+    p, T, w, Ap_in, Ap, As, Ai = inputs(cfg, seed=6)
+    assert p.use_eom and p.GDD != 0
+    As, Ai = grown(As, Ai)
+    s = solver(gpu, cfg)
+    s.upload(w, T, Ap_in, Ap, As, Ai)
+    s.run(0, 2)
+    gp, gs, gi = s.download()
+    O.run(p, w, T, Ap_in, Ap, As, Ai, 0, 2)
+    for g, o in ((gp, Ap), (gs, As), (gi, Ai)):
+        assert rel_l2(g, o) < 1e-9
+
+
+def test_largest_grid_fp32c(gpu):
+    cfg = c5(N=1 << 18, NZ=4, NRT=3)
+    p, T, w, Ap_in, Ap, As, Ai = inputs(cfg, seed=6)
+    As, Ai = grown(As, Ai)
+    s = solver(gpu, cfg, precision=gpu.FP32C)
+    s.upload(w, T, Ap_in, Ap, As, Ai)
+    s.run(0, 1)
+    gp, gs, gi = s.download()
+    O.run(p, w, T, Ap_in, Ap, As, Ai, 0, 1)
+    # FP32-complex fields with FP64 phases and reductions: single-precision level
+    for g, o in ((gp, Ap), (gs, As), (gi, Ai)):
+        assert rel_l2(g, o) < 1e-5
+
+
+@pytest.mark.parametrize("N", [256, 1 << 18])
+def test_linear_step_absorption_only(gpu, N):
+    """With every dispersion term zero the linear step multiplies each field by
+    e^{-alpha dz/2} (cwes3.cuh:112-125): exact for any grid size."""
+    cfg = c2(N=N, NZ=10, NRT=3)
+    P = gpu
+    d = P.derive(to_args(cfg))
+    prm = P.params_from(d, 0)
+    prm.vp = prm.vs = prm.vi = d.vs
+    prm.b2p = prm.b2s = prm.b2i = 0.0
+    T, w = P.grids(d)
+    rng = np.random.default_rng(N)
+    A = [rng.standard_normal(N) + 1j * rng.standard_normal(N) for _ in range(3)]
+    s = P.Solver(prm)
+    s.upload(w, T, P.pump_input(d), *A)
+    s.linear_step()
+    out = s.download()
+    for f, (g, a) in enumerate(zip(out, A)):
+        alpha = (prm.alpha_crp, prm.alpha_crs, prm.alpha_cri)[f]
+        assert rel_l2(g, a * np.exp(-0.5 * alpha * prm.dz)) < 1e-13
