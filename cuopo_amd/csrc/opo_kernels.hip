@@ -17,6 +17,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "opo_device.h"
@@ -449,6 +450,19 @@ __global__ void tick_kernel(OpoInst *__restrict__ insts, TickArgs a) {
 
 // ------------------------------------------------------------------ launchers
 
+// gfx950 has 160 KiB of LDS per CU; a kernel opts in above 64 KiB once per
+// device (the attribute belongs to the device's copy of the function).
+constexpr int kMaxDevices = 64;
+static int opt_in_big_lds(const void *kern, std::once_flag (&flags)[kMaxDevices]) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return (int)hipErrorInvalidDevice;
+  int rc = 0;
+  std::call_once(flags[dev], [&] {
+    rc = (int)hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  });
+  return rc;
+}
+
 template <typename T, int N1, int NF>
 static int launch_col_t(int ct, int count, const OpoInst *insts, const ColArgs &a, hipStream_t s, hipEvent_t e0,
                         hipEvent_t e1) {
@@ -459,14 +473,13 @@ static int launch_col_t(int ct, int count, const OpoInst *insts, const ColArgs &
                    a.fwd_mask == (1 << NF) - 1 && a.load_mask == (1 << NF) - 1 && a.store_mask == (1 << NF) - 1;
   const size_t lds = ((size_t)N1 + 2 * (size_t)NF * ct * N1) * sizeof(V) + 64 * sizeof(double);
   const bool direct = mid && a.direct;
-  static bool big_lds[3] = {false, false, false};  // gfx950 has 160 KiB of LDS; opt in above 64 KiB once
+  static std::once_flag big_lds[3][kMaxDevices];
   const int kv = direct ? 2 : (mid ? 1 : 0);
   auto kern = direct ? &col_stage<T, N1, NF, true, true>
                      : (mid ? &col_stage<T, N1, NF, true, false> : &col_stage<T, N1, NF, false, false>);
-  if (lds > 65536 && !big_lds[kv]) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                              160 * 1024);
-    big_lds[kv] = true;
+  if (lds > 65536) {
+    const int rc = opt_in_big_lds(reinterpret_cast<const void *>(kern), big_lds[kv]);
+    if (rc != 0) return rc;
   }
   if (e0 || e1) hipExtLaunchKernelGGL(kern, grid, dim3(threads), lds, s, e0, e1, 0, insts, a);
   else hipLaunchKernelGGL(kern, grid, dim3(threads), lds, s, insts, a);
@@ -481,12 +494,11 @@ static int launch_row_t(int rt, int count, int tiles_y, const OpoInst *insts, co
   const size_t lds = 2 * (size_t)rt * (N2 + N2 / 8) * sizeof(V);
   dim3 grid((a.N1 / rt) * a.nf * (a.xcd_pack ? 8 : 1), count);
   (void)tiles_y;
-  static bool big_lds[2] = {false, false};
+  static std::once_flag big_lds[2][kMaxDevices];
   auto kern = a.direct ? &row_stage<T, N2, NF, true> : &row_stage<T, N2, NF, false>;
-  if (lds > 65536 && !big_lds[a.direct ? 1 : 0]) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
-                              160 * 1024);
-    big_lds[a.direct ? 1 : 0] = true;
+  if (lds > 65536) {
+    const int rc = opt_in_big_lds(reinterpret_cast<const void *>(kern), big_lds[a.direct ? 1 : 0]);
+    if (rc != 0) return rc;
   }
   if (e0 || e1) hipExtLaunchKernelGGL(kern, grid, dim3(threads), lds, s, e0, e1, 0, insts, a);
   else hipLaunchKernelGGL(kern, grid, dim3(threads), lds, s, insts, a);
