@@ -165,8 +165,8 @@ def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C {HERE}/csrc` "
-                               "(or __graft_entry__.build()); cuopo_amd has no CPU fallback")
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C {HERE}/csrc`; "
+                               "cuopo_amd has no CPU fallback")
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)

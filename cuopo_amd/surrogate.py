@@ -14,7 +14,7 @@ only their sum is fixed by the pump):
 An off point (decayed below threshold) has a zero shape and the power floor.
 Warm fields take the predicted shape at the predicted power with random mode
 phases.  The solver starts from that instead of noise and iterates to its own
-convergence test; the cold start stays the fallback and the judge of the
+convergence test; the cold start stays the fallback and the yardstick of the
 prediction's error.
 
 Data flow: `generate_dataset` runs batched sweeps (opo_batch_*) to steady

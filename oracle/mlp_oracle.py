@@ -1,4 +1,4 @@
-"""FP64 CPU checker for the warm-start MLP (TEST INFRASTRUCTURE ONLY).
+"""FP64 CPU oracle for the warm-start MLP (TEST INFRASTRUCTURE ONLY).
 
 Same network as libcuopo_amd's opo_mlp_*: n_in -> width (GELU) x depth -> n_out,
 flat parameters laid out per layer as W [dout x din] row-major then b [dout].

@@ -1,7 +1,7 @@
 /*
  * opo_oracle.c -- FP64 CPU restatement of cuOPO's SSFM round-trip solver.
  *
- * TEST INFRASTRUCTURE ONLY (the checker).  Only tests/, the package smoke
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, the package smoke
  * check and bench.py's cpu_baseline leg load this library; the product path
  * never does.  Each function cites the reference lines it restates; the
  * header lists what pins each one.  Expressions keep the reference's

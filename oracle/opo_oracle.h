@@ -1,8 +1,9 @@
 /*
  * opo_oracle.h -- FP64 CPU restatement of cuOPO's split-step Fourier round-trip
  * solver.  TEST INFRASTRUCTURE ONLY: tests/, the package smoke check and the
- * CPU-baseline leg of bench.py may call this library, as the checker; the
- * product path (libcuopo_amd.so, the cuOPO executable) never links or calls it.
+ * CPU-baseline leg of bench.py may call this library (to compare results
+ * against, or to time as the CPU baseline); the product path (libcuopo_amd.so,
+ * the cuOPO executable) never links or calls it.
  *
  * Every function cites the reference file:line it restates (paths relative to
  * the cuOPO source tree).  The reference computes in FP32 with cuFFT; this

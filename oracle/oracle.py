@@ -1,7 +1,7 @@
 """ctypes bindings for the FP64 oracle (``oracle/libopo_oracle.so``).
 
 TEST INFRASTRUCTURE ONLY: imported by ``tests/``, the package smoke check and
-the CPU-baseline leg of ``bench.py`` as the checker.  The product path
+the CPU-baseline leg of ``bench.py``, to compare against.  The product path
 (``cuopo_amd``) never imports it.
 
 Fields are numpy ``complex128`` arrays (interleaved re/im, the layout of

@@ -1,4 +1,4 @@
-"""The surrogate MLP's BF16-MFMA kernels against the FP64 CPU checker
+"""The surrogate MLP's BF16-MFMA kernels against the FP64 CPU oracle
 (oracle/mlp_oracle.py) on identical BF16 operands."""
 import numpy as np
 import pytest
@@ -22,7 +22,7 @@ def make(shape, batch, seed):
 
 
 @pytest.mark.parametrize("shape,batch", [((5, 64, 1, 80), 33), ((5, 96, 2, 70), 64), ((5, 256, 4, 4096), 128)])
-def test_forward_matches_fp64_checker(gpu, shape, batch):
+def test_forward_matches_fp64_oracle(gpu, shape, batch):
     """Ragged shapes (not multiples of the 64x64 tile) exercise the MFMA lane
     layout and the tile edges."""
     net, theta, rng = make(shape, batch, 1)
@@ -33,7 +33,7 @@ def test_forward_matches_fp64_checker(gpu, shape, batch):
 
 
 @pytest.mark.parametrize("shape,batch", [((5, 64, 2, 80), 48), ((5, 256, 4, 4096), 128)])
-def test_gradients_match_fp64_checker(gpu, shape, batch):
+def test_gradients_match_fp64_oracle(gpu, shape, batch):
     net, theta, rng = make(shape, batch, 2)
     x = rng.uniform(-1, 1, (batch, shape[0])).astype(np.float32)
     t = rng.standard_normal((batch, shape[3])).astype(np.float32)

@@ -1,4 +1,4 @@
-"""CPU pins for the MLP checker (oracle/mlp_oracle.py) and the surrogate's
+"""CPU pins for the MLP oracle (oracle/mlp_oracle.py) and the surrogate's
 spectrum encodings."""
 import numpy as np
 import pytest
