@@ -50,24 +50,28 @@ def _setup(pt, NZ, max_rt, precision, seed, k, sigma, warm):
     return p, d, (w, T, Ap_in, Ap_in[:N].copy(), As, Ai)
 
 
-def make_batch(points, NZ=100, max_rt=5000, precision=FP32C, seed=1000, sigma=1e-20, warm=None, tiles=(0, 0)):
-    """An uploaded Batch of the sweep points (and the per-point setup)."""
+def make_batch(points, NZ=100, max_rt=5000, precision=FP32C, seed=1000, sigma=1e-20, warm=None, tiles=(0, 0),
+               autotune=False):
+    """An uploaded Batch of the sweep points (and the per-point setup); with
+    autotune, the per-grid-size launch settings are timed and chosen first."""
     made = [_setup(pt, NZ, max_rt, precision, seed, k, sigma, warm) for k, pt in enumerate(points)]
     b = Batch([m[0] for m in made])
     if tiles != (0, 0):
         b.set_tiles(*tiles)
     for k, m in enumerate(made):
         b.upload(k, *m[2])
+    if autotune:
+        b.autotune(iters=5)
     return b, made
 
 
 def run_batched(points, NZ=100, max_rt=5000, check_every=50, tol=1e-6, off_ratio=1e-6, precision=FP32C,
-                seed=1000, sigma=1e-20, warm=None, tiles=(0, 0)):
+                seed=1000, sigma=1e-20, warm=None, tiles=(0, 0), autotune=True):
     """All points in one batch.  Returns a dict with per-point round trips,
     states (1 steady, 2 off, 0 capped, -1 diverged), final powers, the
     device time of the run and the wall time including uploads."""
     t0 = time.perf_counter()
-    b, made = make_batch(points, NZ, max_rt, precision, seed, sigma, warm, tiles)
+    b, made = make_batch(points, NZ, max_rt, precision, seed, sigma, warm, tiles, autotune)
     t1 = time.perf_counter()
     rts, st = b.run_ex(max_rt, check_every=check_every, tol=tol, off_ratio=off_ratio)
     t2 = time.perf_counter()
@@ -78,13 +82,15 @@ def run_batched(points, NZ=100, max_rt=5000, check_every=50, tol=1e-6, off_ratio
 
 
 def run_single_fp64(pt, NZ=100, max_rt=5000, check_every=50, tol=1e-6, off_ratio=1e-6, seed=1000, k=0,
-                    sigma=1e-20):
+                    sigma=1e-20, autotune=True):
     """Plain per-instance FP64 cold start: one solver, round trips in chunks of
     check_every with the same convergence and off tests evaluated on the host
     from the power history (the way a per-point driver would)."""
     p, d, up = _setup(pt, NZ, max_rt, FP64, seed, k, sigma, None)
     s = Solver(p)
     s.upload(*up)
+    if autotune:
+        s.autotune(iters=10)
     t0 = time.perf_counter()
     done, state, prev, first, rprev = 0, 0, 0.0, 0.0, 0.0
     while done < max_rt:
