@@ -43,7 +43,7 @@ def workload_c5(N=1 << 18, NZ=200, NRT=40):
                   Nth=4.0, R_pct=98.0, gdd_pct=100.0, upm=1, md=0.8 / np.pi, fm=4.0, lcr_mm=5.0)
 
 
-TUNING = None  # (col_threads, row_threads, xcd_pack, phase_mode) instead of autotuning
+TUNING = None  # (col_threads, row_threads, xcd_pack, phase_mode[, col_spt, row_spt]) instead of autotuning
 
 
 def log(msg):
@@ -54,12 +54,14 @@ def tune(obj):
     """Autotune a Solver or Batch, or apply the fixed --tuning settings."""
     if TUNING is None:
         return obj.autotune(iters=10)
-    c, r, x, ph = TUNING
+    c, r, x, ph = TUNING[:4]
+    ce, re_ = TUNING[4:6] if len(TUNING) == 6 else (0, 0)
+    kw = dict(col_threads=c, row_threads=r, xcd_pack=x, phase_mode=ph, col_spt=ce, row_spt=re_)
     if hasattr(obj, "count"):
         for t in obj.get_tuning():
-            obj.set_tuning(t["N"], col_threads=c, row_threads=r, xcd_pack=x, phase_mode=ph)
+            obj.set_tuning(t["N"], **kw)
         return obj.get_tuning()
-    obj.set_tuning(col_threads=c, row_threads=r, xcd_pack=x, phase_mode=ph)
+    obj.set_tuning(**kw)
     return obj.get_tuning()
 
 
@@ -232,7 +234,7 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-shapes", action="store_true", help="skip the N=2^18 and batched FP64 sections")
     ap.add_argument("--tuning", default=None,
-                    help="col_threads,row_threads,xcd_pack,phase_mode applied instead of autotuning "
+                    help="col_threads,row_threads,xcd_pack,phase_mode[,col_spt,row_spt] applied instead of autotuning "
                          "(profiling runs: the profiler perturbs the autotuner's timings)")
     ap.add_argument("--only", choices=["c5", "c3"], default=None,
                     help="run only the N=2^18 (c5) or the batched FP64 (c3) section and print its JSON "
@@ -251,8 +253,8 @@ def main():
     global TUNING
     if a.tuning:
         TUNING = tuple(int(v) for v in a.tuning.split(","))
-        if len(TUNING) != 4:
-            raise SystemExit("--tuning wants col_threads,row_threads,xcd_pack,phase_mode")
+        if len(TUNING) not in (4, 6):
+            raise SystemExit("--tuning wants col_threads,row_threads,xcd_pack,phase_mode[,col_spt,row_spt]")
     if a.only:
         print(json.dumps(c5_section() if a.only == "c5" else c3_section()))
         return

@@ -109,7 +109,7 @@ SIGNATURES = {
     "opo_power_history": (_i, [_p, _p, _i]),
     "opo_set_tiles": (_i, [_p, _i, _i]),
     "opo_use_graph": (_i, [_p, _i]),
-    "opo_linear_factors": (_i, [_p, _p, _i, _i, _p]),
+    "opo_linear_factors": (_i, [_p, _p, _i, _i, _i, _p]),
     "opo_set_xcd_pack": (_i, [_p, _i]),
     "opo_autotune": (_i, [_p, _i]),
     "opo_get_tuning": (_i, [_p, _p]),
@@ -153,7 +153,8 @@ _lib = None
 class Tuning(C.Structure):
     """opo_tuning (include/opo.h)."""
     _fields_ = [(n, C.c_int) for n in ("N", "count", "col_threads", "row_threads", "xcd_pack", "phase_mode",
-                                       "col_tile", "row_tile", "factored")] + [("slice_us", C.c_double)]
+                                       "col_spt", "row_spt", "col_tile", "row_tile", "factored")] + \
+        [("slice_us", C.c_double)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -369,8 +370,9 @@ class Solver:
         _check(lib().opo_get_tuning(self._h, C.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, col_threads=0, row_threads=0, xcd_pack=-1, phase_mode=-1):
-        t = Tuning(col_threads=col_threads, row_threads=row_threads, xcd_pack=xcd_pack, phase_mode=phase_mode)
+    def set_tuning(self, col_threads=0, row_threads=0, xcd_pack=-1, phase_mode=-1, col_spt=0, row_spt=0):
+        t = Tuning(col_threads=col_threads, row_threads=row_threads, xcd_pack=xcd_pack, phase_mode=phase_mode,
+                   col_spt=col_spt, row_spt=row_spt)
         _check(lib().opo_set_tuning(self._h, C.byref(t)))
 
     def last_run_ms(self):
@@ -481,9 +483,9 @@ class Batch:
         lib().opo_batch_get_tuning(self._h, C.cast(arr, C.c_void_p), n)
         return [arr[k].as_dict() for k in range(n)]
 
-    def set_tuning(self, N, col_threads=0, row_threads=0, xcd_pack=-1, phase_mode=-1):
+    def set_tuning(self, N, col_threads=0, row_threads=0, xcd_pack=-1, phase_mode=-1, col_spt=0, row_spt=0):
         t = Tuning(N=N, col_threads=col_threads, row_threads=row_threads, xcd_pack=xcd_pack,
-                   phase_mode=phase_mode)
+                   phase_mode=phase_mode, col_spt=col_spt, row_spt=row_spt)
         _check(lib().opo_batch_set_tuning(self._h, C.byref(t)))
 
     def time_stages(self, N, iters=20):

@@ -74,7 +74,7 @@ struct DevMem {
 struct Instance {
   opo_params prm{};
   void *state = nullptr, *lin = nullptr, *gdd = nullptr, *eom = nullptr, *pump = nullptr, *snap = nullptr;
-  void *lfac = nullptr;  // factorised linear phase (used when use_fac)
+  void *lfac[2] = {nullptr, nullptr};  // factorised linear phase for 8 / 4 samples per thread
   bool use_fac = false;
   void *save[3] = {nullptr, nullptr, nullptr};
   double *nl = nullptr;
@@ -84,7 +84,8 @@ struct Instance {
 
 struct Group {
   int N = 0, N1 = 0, N2 = 0, NF = 0, fp32 = 0, lobits = 0;
-  int xl_bits = 0;  // cross-term table split of the factorised linear phase
+  int xl_bits[2] = {0, 0};  // cross-term table split of the factorised phase, 8 / 4 samples per thread
+  int ce = 8, re = 8;        // samples per thread of the column / row FFTs (4 or 8)
   std::vector<int> members;
   std::vector<Instance> inst;
   void *tw1 = nullptr, *tw2 = nullptr, *twlo = nullptr, *twhi = nullptr;
@@ -141,6 +142,10 @@ struct Engine {
 };
 
 size_t esize(const Group &g) { return g.fp32 ? 8 : 16; }
+int eidx(int spt) { return spt == 4 ? 1 : 0; }
+// the kernels' launch bounds (column stage; row stage of N2 points)
+int max_threads(int spt) { return spt == 4 ? 1024 : 512; }
+int max_row_threads(int spt, int N2) { return (spt == 4 && N2 >= 4096) ? 1024 : 512; }
 
 // Linear-phase source of a group.  The factorised tables (opo_phase.cpp) save
 // the N-point table read per field and slice but cost extra loads and
@@ -210,30 +215,33 @@ int floor_pow2(int x) {
   return p;
 }
 
-// Column tile ct (columns per block, threads NF*ct*N1/8) and row tile rt (rows
-// per block, threads rt*N2/8): powers of two dividing N2 / N1, at most 256
-// threads, within the LDS of one CU.
+// Column tile ct (columns per block, threads NF*ct*N1/ce) and row tile rt
+// (rows per block, threads rt*N2/re): powers of two dividing N2 / N1, within
+// the kernels' launch bounds and the LDS of one CU.
 void pick_tiles(Engine &E, Group &g) {
-  const int T1 = g.N1 / 8, T2 = g.N2 / 8;
+  const int T1 = g.N1 / g.ce, T2 = g.N2 / g.re;
   // default column tile: one 128-byte line of columns (8 FP64 / 16 FP32)
   const int colt = g.col_threads ? g.col_threads : E.col_threads;
   int ct = colt ? std::max(1, colt / T1) : (int)(128 / esize(g));
   // default row tile: 64 threads (the autotuner's choice on gfx950 for almost every group)
   int rtn = g.row_threads ? g.row_threads : (E.row_threads ? E.row_threads : 64);
   ct = std::min(floor_pow2(ct), g.N2);
-  while (ct > 1 && g.NF * ct * T1 > 512) ct >>= 1;
+  while (ct > 1 && g.NF * ct * T1 > max_threads(g.ce)) ct >>= 1;
   g.ct = ct;
   g.rt = std::max(1, std::min(floor_pow2(std::max(1, rtn / T2)), g.N1));
+  while (g.rt > 1 && g.rt * T2 > max_row_threads(g.re, g.N2)) g.rt >>= 1;
   while (g.ct > 1 && col_lds(g, g.ct) > kMaxLds) g.ct >>= 1;
   while (g.rt > 1 && row_lds(g, g.rt) > kMaxLds) g.rt >>= 1;
 }
 
 // host-side check of a launch shape against what the kernels assume
 int check_shapes(const Group &g) {
-  const int T1 = g.N1 / 8, T2 = g.N2 / 8;
-  if (!is_pow2(g.ct) || g.N2 % g.ct != 0 || g.NF * g.ct * T1 > 512 || col_lds(g, g.ct) > kMaxLds)
+  if ((g.ce != 4 && g.ce != 8) || (g.re != 4 && g.re != 8))
+    return fail(OPO_ERR_INVALID, "internal: samples per thread must be 4 or 8");
+  const int T1 = g.N1 / g.ce, T2 = g.N2 / g.re;
+  if (!is_pow2(g.ct) || g.N2 % g.ct != 0 || g.NF * g.ct * T1 > max_threads(g.ce) || col_lds(g, g.ct) > kMaxLds)
     return fail(OPO_ERR_INVALID, "internal: column tile does not fit the kernel");
-  if (!is_pow2(g.rt) || g.N1 % g.rt != 0 || g.rt * T2 > 512 || row_lds(g, g.rt) > kMaxLds)
+  if (!is_pow2(g.rt) || g.N1 % g.rt != 0 || g.rt * T2 > max_row_threads(g.re, g.N2) || row_lds(g, g.rt) > kMaxLds)
     return fail(OPO_ERR_INVALID, "internal: row tile does not fit the kernel");
   if (g.N1 < 16 || g.N2 < 16 || g.N1 > 128 || g.N2 > 4096 || (long long)g.N1 * g.N2 != g.N)
     return fail(OPO_ERR_INVALID, "internal: four-step split out of range");
@@ -251,7 +259,8 @@ int make_group(Engine &E, Group &g) {
   const int row_max = g.fp32 ? 4096 : 2048;
   while (g.N2 > row_max) { g.N2 >>= 1; g.N1 <<= 1; }
   g.lobits = (L + 1) / 2;
-  g.xl_bits = std::max(0, (L - 3 + 1) / 2);  // cross-term tables cover e < N/8
+  g.xl_bits[0] = std::max(0, (L - 3 + 1) / 2);  // cross-term tables cover e < N/8
+  g.xl_bits[1] = std::max(0, (L - 2 + 1) / 2);  // ... and e < N/4
   const size_t es = esize(g);
   RCHK(dalloc(g, &g.tw1, g.N1 * es));
   RCHK(dalloc(g, &g.tw2, g.N2 * es));
@@ -316,11 +325,15 @@ int upload_instance(Engine &E, Group &g, int k, const double *w, const double *T
   // The full N-point table per field always; on the reference's grid also the
   // factorised tables (opo_phase.cpp).  group_factored picks one per group.
   {
-    std::vector<double> fac((size_t)2 * NF * opo::lin_factor_len(N, N1, g.xl_bits));
-    I.use_fac = opo::lin_factors(p, w, N1, g.xl_bits, fac.data()) == 1;
-    if (I.use_fac) {
-      if (!I.lfac) RCHK(dalloc(g, &I.lfac, fac.size() / 2 * es));
-      RCHK(upload_c(E, g, I.lfac, reinterpret_cast<const cd *>(fac.data()), fac.size() / 2));
+    I.use_fac = true;
+    for (int k = 0; k < 2 && I.use_fac; ++k) {
+      const int spt = k ? 4 : 8;
+      std::vector<double> fac((size_t)2 * NF * opo::lin_factor_len(N, N1, g.xl_bits[k], spt));
+      I.use_fac = opo::lin_factors(p, w, N1, g.xl_bits[k], spt, fac.data()) == 1;
+      if (I.use_fac) {
+        if (!I.lfac[k]) RCHK(dalloc(g, &I.lfac[k], fac.size() / 2 * es));
+        RCHK(upload_c(E, g, I.lfac[k], reinterpret_cast<const cd *>(fac.data()), fac.size() / 2));
+      }
     }
     std::vector<cd> lin((size_t)NF * N);
     double D[3], b2[3], b3[3], al[3];
@@ -425,7 +438,7 @@ int sync_insts(Engine &E) {
       const opo_params &p = I.prm;
       OpoInst &d = h[k];
       std::memset(&d, 0, sizeof(d));
-      d.state = I.state; d.lin = I.lin; d.lfac = I.use_fac && group_factored(g) ? I.lfac : nullptr; d.gdd = I.gdd; d.eom = p.use_eom ? I.eom : nullptr; d.nl = I.nl;
+      d.state = I.state; d.lin = I.lin; d.lfac = I.use_fac && group_factored(g) ? I.lfac[eidx(g.re)] : nullptr; d.gdd = I.gdd; d.eom = p.use_eom ? I.eom : nullptr; d.nl = I.nl;
       d.pump = I.pump; d.pump_len = I.pump_len;
       for (int f = 0; f < 3; ++f) d.save[f] = E.save_enabled ? I.save[f] : nullptr;
       d.save_first = p.regime == OPO_CW ? p.NRT - p.Nrts : 0;
@@ -462,6 +475,7 @@ int emit_col(Engine &E, Group &g, unsigned ops, int load, int inv, int fwd, int 
   a.snap_every = snap_every;
   a.xcd_pack = g.xcd_pack >= 0 ? g.xcd_pack : E.xcd_pack;
   a.swz = !a.xcd_pack && g.ct * esize(g) < 128 && (g.N2 / g.ct) % 8 == 0;
+  a.spt = g.ce;
   if (E.solver) {
     const Instance &I = g.inst[0];
     a.direct = 1;
@@ -483,12 +497,13 @@ int emit_row(Engine &E, Group &g, bool gdd, int mask, hipEvent_t e0 = nullptr, h
     if ((mask >> f) & 1) fl[nf++] = f;
   a.nf = nf; a.f0 = fl[0]; a.f1 = fl[1]; a.f2 = fl[2];
   a.xcd_pack = g.xcd_pack >= 0 ? g.xcd_pack : E.xcd_pack;
-  a.xl_bits = g.xl_bits;
+  a.xl_bits = g.xl_bits[eidx(g.re)];
+  a.spt = g.re;
   if (E.solver) {
     const Instance &I = g.inst[0];
     a.direct = 1;
     a.d_state = I.state; a.d_lin = I.lin; a.d_gdd = I.gdd;
-    a.d_lfac = I.use_fac && group_factored(g) ? I.lfac : nullptr;
+    a.d_lfac = I.use_fac && group_factored(g) ? I.lfac[eidx(g.re)] : nullptr;
   }
   int e = opo::launch_row(g.fp32, g.N2, g.NF, g.rt, (int)g.members.size(), 0, g.d_insts, a, g.cur, e0, e1);
   if (e != 0) return fail(OPO_ERR_HIP, std::string("row_stage launch: ") + hipGetErrorString((hipError_t)e));
@@ -1168,6 +1183,8 @@ void get_tuning(const Engine &E, const Group &g, opo_tuning *t) {
   t->row_threads = g.row_threads ? g.row_threads : E.row_threads;
   t->xcd_pack = g.xcd_pack >= 0 ? g.xcd_pack : E.xcd_pack;
   t->phase_mode = g.phase_mode;
+  t->col_spt = g.ce;
+  t->row_spt = g.re;
   t->col_tile = g.ct;
   t->row_tile = g.rt;
   t->factored = 0;
@@ -1180,20 +1197,34 @@ int set_tuning(Engine &E, Group &g, const opo_tuning *t) {
     if (v != 0 && v != 32 && v != 64 && v != 128 && v != 256 && v != 512)
       return fail(OPO_ERR_INVALID, "tile threads in {0,32,64,128,256,512}");
   if (t->phase_mode < -1 || t->phase_mode > 1) return fail(OPO_ERR_INVALID, "phase_mode in {-1,0,1}");
+  for (int v : {t->col_spt, t->row_spt})
+    if (v != 0 && v != 4 && v != 8) return fail(OPO_ERR_INVALID, "samples per thread in {0,4,8}");
+  const Group old = g;  // restored if the settings do not fit the kernels
   g.col_threads = t->col_threads;
   g.row_threads = t->row_threads;
   g.xcd_pack = t->xcd_pack < 0 ? -1 : (t->xcd_pack ? 1 : 0);
   g.phase_mode = t->phase_mode;
+  g.ce = t->col_spt ? t->col_spt : 8;
+  g.re = t->row_spt ? t->row_spt : 8;
   g.tuned_us = 0;
   pick_tiles(E, g);
-  RCHK(check_shapes(g));
+  const int rc = check_shapes(g);
+  if (rc != OPO_OK) {
+    g.col_threads = old.col_threads; g.row_threads = old.row_threads; g.xcd_pack = old.xcd_pack;
+    g.phase_mode = old.phase_mode; g.ce = old.ce; g.re = old.re; g.tuned_us = old.tuned_us;
+    g.ct = old.ct; g.rt = old.rt;
+    return rc;
+  }
   E.graph_key = -1;
   E.inst_dirty = true;
   return OPO_OK;
 }
 
-// Time every candidate (column tile, row tile, XCD packing, phase source) of
-// each group with engine_time_stages and keep the fastest.  The stage runs
+// Time the candidates of each group with engine_time_stages and keep the
+// fastest: for each XCD packing, the column settings (tile of 1/2, 1 or 2
+// cache lines x 4 or 8 samples per thread) are timed with the row settings
+// fixed, then the row settings (64..512-thread tiles x 4 or 8 samples per
+// thread x phase source) with the best column settings.  The stage runs
 // mutate the fields, so each group's state is saved before and restored after.
 int engine_autotune(Engine &E, int iters) {
   for (auto &g : E.groups) {
@@ -1209,35 +1240,53 @@ int engine_autotune(Engine &E, int iters) {
     HIPCHK(hipStreamSynchronize(E.stream));
     opo_tuning base;
     get_tuning(E, g, &base);
-    const int T1 = g.N1 / 8;
     const int ct0 = (int)(128 / esize(g));
-    std::vector<int> cols, rows = {64, 128, 256, 512};
-    for (int ct : {ct0 / 2, ct0, ct0 * 2}) {
-      const int th = ct * T1;
-      if (th >= 32 && th <= 512 && (th & (th - 1)) == 0) cols.push_back(th);
-    }
-    if (cols.empty()) cols.push_back(0);
     std::vector<int> xcds = {0};
     if ((long long)n * g.N <= (1LL << 16)) xcds.push_back(1);
     bool can_fac = true;
     for (auto &I : g.inst) can_fac &= I.use_fac;
     std::vector<int> phases = {0};
     if (can_fac) phases.push_back(1);
+    auto time_it = [&](const opo_tuning &t, double *cu, double *ru) -> int {
+      opo_tuning tt = t;
+      if (set_tuning(E, g, &tt) != OPO_OK) return 1;  // shape does not fit: skip
+      RCHK(sync_insts(E));
+      RCHK(engine_time_stages(E, g, iters, cu, ru));
+      return OPO_OK;
+    };
     opo_tuning best = base;
     double best_us = 1e30;
-    std::vector<std::pair<int, int>> seen;
-    for (int c : cols)
-      for (int r : rows)
-        for (int x : xcds)
+    for (int x : xcds) {
+      opo_tuning t = base;
+      t.xcd_pack = x;
+      t.row_threads = 64; t.row_spt = 8; t.phase_mode = -1;
+      double bc = 1e30;
+      opo_tuning tc = t;
+      for (int spt : {8, 4})
+        for (int ct : {ct0 / 2, ct0, ct0 * 2}) {
+          const int th = ct * (g.N1 / spt);
+          if (th < 32 || th > 512 || (th & (th - 1))) continue;
+          opo_tuning c = t;
+          c.col_threads = th; c.col_spt = spt;
+          double cu = 0, ru = 0;
+          const int rc = time_it(c, &cu, &ru);
+          if (rc < 0) return rc;
+          if (rc == OPO_OK && cu < bc) { bc = cu; tc = c; }
+        }
+      double br = 1e30;
+      opo_tuning tr = tc;
+      for (int spt : {8, 4})
+        for (int r : {64, 128, 256, 512})
           for (int ph : phases) {
-            opo_tuning t = base;
-            t.col_threads = c; t.row_threads = r; t.xcd_pack = x; t.phase_mode = ph;
-            if (set_tuning(E, g, &t) != OPO_OK) continue;
-            RCHK(sync_insts(E));
+            opo_tuning c = tc;
+            c.row_threads = r; c.row_spt = spt; c.phase_mode = ph;
             double cu = 0, ru = 0;
-            RCHK(engine_time_stages(E, g, iters, &cu, &ru));
-            if (cu + ru < best_us) { best_us = cu + ru; best = t; }
+            const int rc = time_it(c, &cu, &ru);
+            if (rc < 0) return rc;
+            if (rc == OPO_OK && ru < br) { br = ru; tr = c; }
           }
+      if (bc + br < best_us) { best_us = bc + br; best = tr; }
+    }
     RCHK(set_tuning(E, g, &best));
     g.tuned_us = best_us;
     for (int k = 0; k < n; ++k)

@@ -1,7 +1,7 @@
 // opo_device.h -- device-side building blocks for the gfx950 SSFM kernels:
 // complex arithmetic on double2/float2, radix-2/4/8 DFT butterflies and a
-// Stockham FFT executed by a "team" of M/8 threads that each hold 8 samples in
-// registers and exchange through LDS.
+// Stockham FFT executed by a "team" of M/E threads that each hold E = 4 or 8
+// samples in registers and exchange through LDS.
 //
 // Sign convention: S = +1 computes sum_j x_j e^{+2 pi i jk/M} (the transform
 // cuOPO reaches with CUFFT_INVERSE, src/headers/cwes3.cuh:175), S = -1 the
@@ -34,9 +34,13 @@ template <int S, typename V> __device__ __forceinline__ V mul_si(V a) {
 template <int S, typename V> __device__ __forceinline__ V tw_sign(V w) { return S > 0 ? w : cconj(w); }
 
 constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x >> 1); }
-constexpr int npasses(int M) { return (ilog2c(M) + 2) / 3; }
-constexpr int radix_at(int M, int q) { return q < ilog2c(M) / 3 ? 8 : (1 << (ilog2c(M) - 3 * (ilog2c(M) / 3))); }
-constexpr int p_at(int q) { return 1 << (3 * q); }
+// passes of radix E (E = 4 or 8 samples per thread), the last one of radix
+// 2^(log2 M mod log2 E) when log2 M is not a multiple of log2 E
+constexpr int npasses(int M, int E) { return (ilog2c(M) + ilog2c(E) - 1) / ilog2c(E); }
+constexpr int radix_at(int M, int q, int E) {
+  return q < ilog2c(M) / ilog2c(E) ? E : (1 << (ilog2c(M) - ilog2c(E) * (ilog2c(M) / ilog2c(E))));
+}
+constexpr int p_at(int q, int E) { return 1 << (ilog2c(E) * q); }
 
 // ---- butterflies (natural-order in, natural-order out) ----
 template <int S, typename V> __device__ __forceinline__ void dft2(V* u) {
@@ -68,18 +72,19 @@ template <int R, int S, typename V> __device__ __forceinline__ void dftr(V* u) {
   else if constexpr (R == 2) dft2<S>(u);
 }
 
-// Team FFT of length M: a team of M/8 threads, each holding 8 samples in
-// registers, runs radix-8 Stockham passes (a radix-2 or -4 pass last when
-// log2 M is not a multiple of 3).  On entry v[m] = x[t + m*M/8]; on exit
-// v[m] = X[t + m*M/8].  Passes exchange through LDS; the last pass's outputs
+// Team FFT of length M: a team of M/E threads, each holding E samples in
+// registers (E = 8: radix-8 passes; E = 4: radix-4 passes, twice the threads
+// for latency-bound shapes), with a radix-2 or -4 pass last when log2 M is
+// not a multiple of log2 E.  On entry v[m] = x[t + m*M/E]; on exit
+// v[m] = X[t + m*M/E].  Passes exchange through LDS; the last pass's outputs
 // already sit in the natural distribution (output i + j*M/R belongs to the
 // thread that read input i + j*M/R), so it runs in registers and an FFT of
-// npasses(M) passes costs npasses(M) - 1 exchanges.
+// npasses(M, E) passes costs npasses(M, E) - 1 exchanges.
 //
 // Team element e lives at buf[gbase + lidx(e)], lidx(e) = e*es (PAD false,
 // column tiles: es interleaved columns) or e + e/8 (PAD true, contiguous rows:
-// one pad slot per 8 samples makes the stride-8 writes of the first pass hit
-// distinct banks).  The LDS region holds two ping-pong buffers `bufstride`
+// one pad slot per 8 samples spreads the strided writes of the first pass over
+// the banks).  The LDS region holds two ping-pong buffers `bufstride`
 // elements apart; `ph` (block-uniform) alternates them across every exchange
 // of the kernel, so one barrier per exchange suffices: a buffer is rewritten
 // only after the barrier that follows the next exchange's writes.  `on` (per
@@ -93,20 +98,20 @@ template <int R, int S, typename V> __device__ __forceinline__ void dftr(V* u) {
 template <bool PAD> __device__ __forceinline__ int lidx(int e, int es) { return PAD ? e + (e >> 3) : e * es; }
 
 // per-thread twiddles of passes 1 .. Q-1 (pass 0 has none)
-constexpr int tw_off(int M, int Q) {
-  return Q <= 1 ? 0 : tw_off(M, Q - 1) + (radix_at(M, Q - 1) - 1) * (8 / radix_at(M, Q - 1));
+constexpr int tw_off(int M, int Q, int E) {
+  return Q <= 1 ? 0 : tw_off(M, Q - 1, E) + (radix_at(M, Q - 1, E) - 1) * (E / radix_at(M, Q - 1, E));
 }
-constexpr int tw_count(int M) { return tw_off(M, npasses(M)); }
+constexpr int tw_count(int M, int E) { return tw_off(M, npasses(M, E), E); }
 
-template <int M, int Q, int S, bool PAD, bool TREG, typename V>
+template <int M, int Q, int S, bool PAD, bool TREG, int E, typename V>
 __device__ __forceinline__ void fft_pass(V *v, bool on, V *buf0, int &ph, int t, int gbase, int es, int bufstride,
                                          const V *tws) {
-  constexpr int R = radix_at(M, Q);
-  constexpr int P = p_at(Q);
-  constexpr int T = M / 8;
-  constexpr int NB = 8 / R;
-  constexpr bool LAST = Q == npasses(M) - 1;
-  constexpr int TOFF = tw_off(M, Q);  // constant: keeps the register set statically indexed
+  constexpr int R = radix_at(M, Q, E);
+  constexpr int P = p_at(Q, E);
+  constexpr int T = M / E;
+  constexpr int NB = E / R;
+  constexpr bool LAST = Q == npasses(M, E) - 1;
+  constexpr int TOFF = tw_off(M, Q, E);  // constant: keeps the register set statically indexed
   V *buf = buf0 + (ph & 1) * bufstride;
   if (on) {
 #pragma unroll
@@ -138,38 +143,38 @@ __device__ __forceinline__ void fft_pass(V *v, bool on, V *buf0, int &ph, int t,
     __syncthreads();
     if (on) {
 #pragma unroll
-      for (int m = 0; m < 8; ++m) v[m] = buf[gbase + lidx<PAD>(t + m * T, es)];
+      for (int m = 0; m < E; ++m) v[m] = buf[gbase + lidx<PAD>(t + m * T, es)];
     }
     ++ph;
   }
 }
 
-template <int M, int S, bool PAD, bool TREG, int Q = 0, typename V>
+template <int M, int S, bool PAD, bool TREG, int E, int Q = 0, typename V>
 __device__ __forceinline__ void team_fft(V *v, bool on, V *buf0, int &ph, int t, int gbase, int es, int bufstride,
                                          const V *tws) {
-  if constexpr (Q < npasses(M)) {
-    fft_pass<M, Q, S, PAD, TREG>(v, on, buf0, ph, t, gbase, es, bufstride, tws);
-    team_fft<M, S, PAD, TREG, Q + 1>(v, on, buf0, ph, t, gbase, es, bufstride, tws);
+  if constexpr (Q < npasses(M, E)) {
+    fft_pass<M, Q, S, PAD, TREG, E>(v, on, buf0, ph, t, gbase, es, bufstride, tws);
+    team_fft<M, S, PAD, TREG, E, Q + 1>(v, on, buf0, ph, t, gbase, es, bufstride, tws);
   }
 }
 
 // This thread's twiddles of every pass, from the M-point root table `g`
-// (global memory), into treg[tw_count(M)]; both transform directions use them.
-template <int M, int Q = 1, typename V>
+// (global memory), into treg[tw_count(M, E)]; both transform directions use them.
+template <int M, int E, int Q = 1, typename V>
 __device__ __forceinline__ void load_twiddles(V *treg, int t, const V *__restrict__ g) {
-  if constexpr (Q < npasses(M)) {
-    constexpr int R = radix_at(M, Q);
-    constexpr int P = p_at(Q);
-    constexpr int T = M / 8;
-    constexpr int NB = 8 / R;
-    constexpr int TOFF = tw_off(M, Q);
+  if constexpr (Q < npasses(M, E)) {
+    constexpr int R = radix_at(M, Q, E);
+    constexpr int P = p_at(Q, E);
+    constexpr int T = M / E;
+    constexpr int NB = E / R;
+    constexpr int TOFF = tw_off(M, Q, E);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const int k = (t + b * T) & (P - 1);
 #pragma unroll
       for (int j = 1; j < R; ++j) treg[TOFF + b * (R - 1) + j - 1] = g[j * k * (M / (P * R))];
     }
-    load_twiddles<M, Q + 1>(treg, t, g);
+    load_twiddles<M, E, Q + 1>(treg, t, g);
   }
 }
 

@@ -34,7 +34,8 @@ enum ColOp : unsigned {
 struct OpoInst {
   void *state;             // NF*N complex
   const void *lin;         // NF*N complex: e^{i phase}*atten/N, permuted (p, s, i); null when lfac is used
-  const void *lfac;        // factorised phase: per field Z (8*N1), S (N2), Xhi (N/8 >> L), Xlo (2^L); or null
+  const void *lfac;        // factorised phase for the row FFT's samples per thread E: per field
+                           // Z (E*N1), S (N2), Xhi (N/E >> L), Xlo (2^L); or null
   const void *gdd;         // 2N complex (signal, idler): e^{i w^2 GDD/2}/N, permuted; null if unused
   const void *eom;         // N complex: e^{i m sin(2 pi fpm T)}, time order; null if unused
   const double *nl;        // NZ*16 doubles: 8 complex FP64 coefficients per slice
@@ -77,6 +78,7 @@ struct ColArgs {
   int snap_every;    // OP_SNAP fires when (rt + 1) % snap_every == 0
   int xcd_pack;      // 1: grid is 8x wider and only blocks with blockIdx.x % 8 == 0 work
   int swz;           // 1: XCD-contiguous tile order (tiles narrower than a cache line)
+  int spt;           // samples per thread of the column FFT (4 or 8)
   // single always-active instance (solver): the slice-interior kernel takes
   // these instead of reading the instance descriptor first
   int direct;
@@ -92,6 +94,7 @@ struct RowArgs {
   int use_gdd;
   int xcd_pack;      // as in ColArgs
   int xl_bits;       // L of the cross-term tables of OpoInst::lfac
+  int spt;           // samples per thread of the row FFT (4 or 8)
   int direct;        // as in ColArgs: pointers below instead of the descriptor
   void *d_state;
   const void *d_lin, *d_lfac, *d_gdd;
@@ -117,8 +120,8 @@ int launch_copy(const void *src, void *dst, long long n, void *stream, void *ev0
 // linear-step phase coefficients (returns NF) and its factorisation (host)
 int lin_coeffs(const opo_params &p, double D[3], double b2[3], double b3[3], double al[3]);
 bool affine_grid(const double *w, int N, double &a0, double &a1, double &s);
-size_t lin_factor_len(int N, int N1, int xl_bits);
-int lin_factors(const opo_params &p, const double *w, int N1, int xl_bits, double *out);
+size_t lin_factor_len(int N, int N1, int xl_bits, int E);
+int lin_factors(const opo_params &p, const double *w, int N1, int xl_bits, int E, double *out);
 
 int launch_tick(OpoInst *insts, const TickArgs &a, void *stream);
 

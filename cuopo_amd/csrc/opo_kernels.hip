@@ -104,17 +104,17 @@ __device__ __forceinline__ void nl_half_pt(V *xp, V *xs, V *xi, const NlCoef<NF,
 }
 
 // Column stage.  The block holds CT columns of every field; the threads are
-// split into NF groups of FT = CT*N1/8 (one wave per field at FT = 64), and
+// split into NF groups of FT = CT*N1/E (E = 4 or 8 samples per thread), and
 // group f runs field f's column FFTs, so the fields' transforms proceed on
 // different SIMDs at once.  The time-domain samples of all fields meet in LDS
-// for the pointwise program, which every group evaluates for its 8 samples;
-// each group keeps its own field.  MID compiles the slice-interior variant
-// only (inverse -> NL end -> NL start -> forward).
-template <typename T, int N1, int NF, bool MID, bool DIRECT>
-__global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ insts, ColArgs a) {
+// for the pointwise program; each group keeps its own field.  MID compiles
+// the slice-interior variant only (inverse -> NL end -> NL start -> forward).
+template <typename T, int N1, int NF, bool MID, bool DIRECT, int E>
+__global__ __launch_bounds__(E == 4 ? 1024 : 512) void col_stage(const OpoInst *__restrict__ insts, ColArgs a) {
   static_assert(MID || !DIRECT, "direct arguments serve the slice-interior kernel");
+  static_assert(E == 4 || E == 8, "4 or 8 samples per thread");
   using V = typename vec2<T>::type;
-  constexpr int T1 = N1 / 8;
+  constexpr int T1 = N1 / E;
   extern __shared__ unsigned char smem_raw[];
   V *ltw = reinterpret_cast<V *>(smem_raw);  // N1-point root table
   V *lds = ltw + N1;                         // 2 buffers x NF fields x CT*N1
@@ -165,37 +165,37 @@ __global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ ins
     const V *tw1 = reinterpret_cast<const V *>(a.tw1);
     for (int e = tid; e < N1; e += blockDim.x) ltw[e] = tw1[e];
   }
-  V v[8];
+  V v[E];
   const bool ld = (load_mask >> f) & 1;
 #pragma unroll
-  for (int m = 0; m < 8; ++m) v[m] = ld ? st[f * N + (long long)N2 * (t + m * T1) + n2] : mk<V>(0, 0);
+  for (int m = 0; m < E; ++m) v[m] = ld ? st[f * N + (long long)N2 * (t + m * T1) + n2] : mk<V>(0, 0);
   // four-step twiddles W_m = e^{2 pi i n2 k1 / N} for this thread's
   // k1 = t + m*T1, from two-level root tables: X(e) = hi[e >> lobits] lo[e & mask]
-  V W[8];
+  V W[E];
   if (any_fft) {
     const V *twlo = reinterpret_cast<const V *>(a.twlo);
     const V *twhi = reinterpret_cast<const V *>(a.twhi);
     const unsigned lm = (1u << a.lobits) - 1u;
     if constexpr (std::is_same<T, double>::value) {
-      // W_m = X(n2 t) X(n2 T1)^m, the powers by a product tree (at most three
-      // complex products per twiddle; the base's rounding grows at most 7x,
-      // ~1e-15 in FP64): 4 gathers per thread instead of 16
+      // W_m = X(n2 t) X(n2 T1)^m, the powers from b, b^2, b^4 by the bits of m
+      // (at most three complex products per twiddle; the base's rounding
+      // grows at most 7x, ~1e-15 in FP64): 4 gathers per thread instead of 2E
       const unsigned e0 = (unsigned)(n2 * t) & nmask, e1 = (unsigned)(n2 * T1) & nmask;
       const V x0 = cmul(twhi[e0 >> a.lobits], twlo[e0 & lm]);
       const V b1 = cmul(twhi[e1 >> a.lobits], twlo[e1 & lm]);
       const V b2 = cmul(b1, b1), b4 = cmul(b2, b2);
-      W[0] = x0;
-      W[1] = cmul(x0, b1);
-      W[2] = cmul(x0, b2);
-      W[3] = cmul(W[2], b1);
-      W[4] = cmul(x0, b4);
-      W[5] = cmul(W[4], b1);
-      W[6] = cmul(W[4], b2);
-      W[7] = cmul(W[6], b1);
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        V w = x0;
+        if (m & 4) w = cmul(w, b4);
+        if (m & 2) w = cmul(w, b2);
+        if (m & 1) w = cmul(w, b1);
+        W[m] = w;
+      }
     } else {
       // FP32: the same growth would cost accuracy, so each twiddle is looked up
 #pragma unroll
-      for (int m = 0; m < 8; ++m) {
+      for (int m = 0; m < E; ++m) {
         const unsigned e = (unsigned)(n2 * (t + m * T1)) & nmask;
         W[m] = cmul(twhi[e >> a.lobits], twlo[e & lm]);
       }
@@ -209,14 +209,14 @@ __global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ ins
   if (inv_mask) {
     if (inv) {
 #pragma unroll
-      for (int m = 0; m < 8; ++m) v[m] = cmul(v[m], cconj(W[m]));
+      for (int m = 0; m < E; ++m) v[m] = cmul(v[m], cconj(W[m]));
     }
-    team_fft<N1, -1, false, false>(v, inv, mine0, ph, t, c, CT, bufstride, ltw);
+    team_fft<N1, -1, false, false, E>(v, inv, mine0, ph, t, c, CT, bufstride, ltw);
   }
   // ---- all fields meet in the next LDS buffer ----
   V *xb = lds + (ph & 1) * bufstride;
 #pragma unroll
-  for (int m = 0; m < 8; ++m) xb[f * fstride + c + (t + m * T1) * CT] = v[m];
+  for (int m = 0; m < E; ++m) xb[f * fstride + c + (t + m * T1) * CT] = v[m];
   __syncthreads();
   ++ph;
   if constexpr (MID) {
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ ins
     // samples m = f (mod NF), in place (each sample belongs to one thread),
     // then every thread reads its own field back.
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
+    for (int m = 0; m < E; ++m) {
       if (m % NF == f) {
         const int o = c + (t + m * T1) * CT;
         V xp[1] = {xb[o]}, xs[1] = {xb[fstride + o]};
@@ -238,9 +238,9 @@ __global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ ins
     }
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < 8; ++m) v[m] = xb[f * fstride + c + (t + m * T1) * CT];
+    for (int m = 0; m < E; ++m) v[m] = xb[f * fstride + c + (t + m * T1) * CT];
   } else {
-  // ---- pointwise program on time samples n = N2*(t + m*T1) + n2, in two
+  // ---- pointwise program on time samples n = N2*(t + m*T1) + n2, in
   //      chunks of 4 samples to bound the live registers ----
   int slot = -1, rp = 0;
   bool snap = false;
@@ -319,7 +319,7 @@ __global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ ins
     for (int q = 0; q < 4; ++q) v[hb + q] = pick(f, xp[q], xs[q], xi[q]);
   };
   chunk(std::integral_constant<int, 0>{});
-  chunk(std::integral_constant<int, 4>{});
+  if constexpr (E == 8) chunk(std::integral_constant<int, 4>{});
   {
     if (ops & OP_POWER) {
       // deterministic block reduction: wave shuffle tree, then waves in order;
@@ -343,22 +343,26 @@ __global__ __launch_bounds__(512) void col_stage(const OpoInst *__restrict__ ins
   // ---- forward column transform of this group's field ----
   const bool fw = (fwd_mask >> f) & 1;
   if (fwd_mask) {
-    team_fft<N1, +1, false, false>(v, fw, mine0, ph, t, c, CT, bufstride, ltw);
+    team_fft<N1, +1, false, false, E>(v, fw, mine0, ph, t, c, CT, bufstride, ltw);
     if (fw) {
 #pragma unroll
-      for (int m = 0; m < 8; ++m) v[m] = cmul(v[m], W[m]);
+      for (int m = 0; m < E; ++m) v[m] = cmul(v[m], W[m]);
     }
   }
   if ((store_mask >> f) & 1) {
 #pragma unroll
-    for (int m = 0; m < 8; ++m) st[f * N + (long long)N2 * (t + m * T1) + n2] = v[m];
+    for (int m = 0; m < E; ++m) st[f * N + (long long)N2 * (t + m * T1) + n2] = v[m];
   }
 }
 
-template <typename T, int N2, int NF, bool DIRECT>
-__global__ __launch_bounds__(512) void row_stage(const OpoInst *__restrict__ insts, RowArgs a) {
+// threads per row-stage block: a 4096-point row at 4 samples per thread needs 1024
+constexpr int row_max_threads(int N2, int E) { return (E == 4 && N2 >= 4096) ? 1024 : 512; }
+
+template <typename T, int N2, int NF, bool DIRECT, int E>
+__global__ __launch_bounds__(row_max_threads(N2, E)) void row_stage(const OpoInst *__restrict__ insts, RowArgs a) {
+  static_assert(E == 4 || E == 8, "4 or 8 samples per thread");
   using V = typename vec2<T>::type;
-  constexpr int T2 = N2 / 8;
+  constexpr int T2 = N2 / E;
   constexpr int NP2 = N2 + N2 / 8;  // padded row in LDS
   extern __shared__ unsigned char smem_raw[];
   V *lds = reinterpret_cast<V *>(smem_raw);  // 2 buffers x RT padded rows
@@ -381,7 +385,7 @@ __global__ __launch_bounds__(512) void row_stage(const OpoInst *__restrict__ ins
   const long long N = (long long)a.N1 * N2;
   V *row = reinterpret_cast<V *>(DIRECT ? a.d_state : I->state) + f * N + (long long)N2 * k1;
 
-  V v[8], ph_tab[8], treg[tw_count(N2) > 0 ? tw_count(N2) : 1];
+  V v[E], ph_tab[E], treg[tw_count(N2, E) > 0 ? tw_count(N2, E) : 1];
   const V *fac = reinterpret_cast<const V *>(DIRECT ? a.d_lfac : I->lfac);
   const bool use_fac = !a.use_gdd && fac;
   const V *tab = a.use_gdd ? reinterpret_cast<const V *>(DIRECT ? a.d_gdd : I->gdd) + (f - 1) * N
@@ -390,37 +394,37 @@ __global__ __launch_bounds__(512) void row_stage(const OpoInst *__restrict__ ins
   // the row, then the twiddles
   V xr = mk<V>(0, 0);
   if (use_fac) {
-    // factorised phase Z[8 k1 + m] S[t + m T2] X(k1 t), X = Xhi Xlo
+    // factorised phase Z[E k1 + m] S[t + m T2] X(k1 t), X = Xhi Xlo
     const int L = a.xl_bits;
-    const long long FB = 8LL * a.N1 + N2 + ((N / 8) >> L) + (1LL << L);
+    const long long FB = (long long)E * a.N1 + N2 + ((N / E) >> L) + (1LL << L);
     const V *F = fac + f * FB;
-    const V *S = F + 8 * a.N1;
+    const V *S = F + E * a.N1;
     const V *XH = S + N2;
-    const V *XL = XH + ((N / 8) >> L);
+    const V *XL = XH + ((N / E) >> L);
     const unsigned e = (unsigned)k1 * (unsigned)t;
     xr = cmul(XH[e >> L], XL[e & ((1u << L) - 1u)]);
 #pragma unroll
-    for (int m = 0; m < 8; ++m) ph_tab[m] = cmul(F[8 * k1 + m], S[t + m * T2]);
+    for (int m = 0; m < E; ++m) ph_tab[m] = cmul(F[E * k1 + m], S[t + m * T2]);
   } else {
     tab += (long long)N2 * k1;
 #pragma unroll
-    for (int m = 0; m < 8; ++m) ph_tab[m] = tab[t + m * T2];
+    for (int m = 0; m < E; ++m) ph_tab[m] = tab[t + m * T2];
   }
 #pragma unroll
-  for (int m = 0; m < 8; ++m) v[m] = row[t + m * T2];
-  load_twiddles<N2>(treg, t, reinterpret_cast<const V *>(a.tw2));
+  for (int m = 0; m < E; ++m) v[m] = row[t + m * T2];
+  load_twiddles<N2, E>(treg, t, reinterpret_cast<const V *>(a.tw2));
   if (use_fac) {
 #pragma unroll
-    for (int m = 0; m < 8; ++m) ph_tab[m] = cmul(ph_tab[m], xr);
+    for (int m = 0; m < E; ++m) ph_tab[m] = cmul(ph_tab[m], xr);
   }
   int ph = 0;
   const int bufstride = RT * NP2;
-  team_fft<N2, +1, true, true>(v, true, lds, ph, t, r * NP2, 1, bufstride, treg);
+  team_fft<N2, +1, true, true, E>(v, true, lds, ph, t, r * NP2, 1, bufstride, treg);
 #pragma unroll
-  for (int m = 0; m < 8; ++m) v[m] = cmul(v[m], ph_tab[m]);
-  team_fft<N2, -1, true, true>(v, true, lds, ph, t, r * NP2, 1, bufstride, treg);
+  for (int m = 0; m < E; ++m) v[m] = cmul(v[m], ph_tab[m]);
+  team_fft<N2, -1, true, true, E>(v, true, lds, ph, t, r * NP2, 1, bufstride, treg);
 #pragma unroll
-  for (int m = 0; m < 8; ++m) row[t + m * T2] = v[m];
+  for (int m = 0; m < E; ++m) row[t + m * T2] = v[m];
 }
 
 // One thread per instance: reduce the column-tile partials in order, record
@@ -483,22 +487,53 @@ static int opt_in_big_lds(const void *kern, std::once_flag (&flags)[kMaxDevices]
   return rc;
 }
 
-template <typename T, int N1, int NF>
-static int launch_col_t(int ct, int count, const OpoInst *insts, const ColArgs &a, hipStream_t s, hipEvent_t e0,
+template <int NF> static bool col_is_mid(const ColArgs &a) {
+  return a.ops == (OP_NL_END | OP_NL_START) && a.inv_mask == (1 << NF) - 1 && a.fwd_mask == (1 << NF) - 1 &&
+         a.load_mask == (1 << NF) - 1 && a.store_mask == (1 << NF) - 1;
+}
+
+template <typename T, int N1, int NF, int E>
+static int launch_col_e(int ct, int count, const OpoInst *insts, const ColArgs &a, hipStream_t s, hipEvent_t e0,
                         hipEvent_t e1) {
   using V = typename vec2<T>::type;
-  const int threads = NF * ct * (N1 / 8);
   dim3 grid((a.N2 / ct) * (a.xcd_pack ? 8 : 1), count);
-  const bool mid = a.ops == (OP_NL_END | OP_NL_START) && a.inv_mask == (1 << NF) - 1 &&
-                   a.fwd_mask == (1 << NF) - 1 && a.load_mask == (1 << NF) - 1 && a.store_mask == (1 << NF) - 1;
+  const bool mid = col_is_mid<NF>(a);
   const size_t lds = ((size_t)N1 + 2 * (size_t)NF * ct * N1) * sizeof(V) + 64 * sizeof(double);
   const bool direct = mid && a.direct;
   static std::once_flag big_lds[3][kMaxDevices];
   const int kv = direct ? 2 : (mid ? 1 : 0);
-  auto kern = direct ? &col_stage<T, N1, NF, true, true>
-                     : (mid ? &col_stage<T, N1, NF, true, false> : &col_stage<T, N1, NF, false, false>);
+  // the general (boundary) variant always runs 8 samples per thread
+  auto kern = direct ? &col_stage<T, N1, NF, true, true, E>
+                     : (mid ? &col_stage<T, N1, NF, true, false, E> : &col_stage<T, N1, NF, false, false, 8>);
+  const int threads = NF * ct * (N1 / (mid ? E : 8));
   if (lds > 65536) {
     const int rc = opt_in_big_lds(reinterpret_cast<const void *>(kern), big_lds[kv]);
+    if (rc != 0) return rc;
+  }
+  if (e0 || e1) hipExtLaunchKernelGGL(kern, grid, dim3(threads), lds, s, e0, e1, 0, insts, a);
+  else hipLaunchKernelGGL(kern, grid, dim3(threads), lds, s, insts, a);
+  return (int)hipGetLastError();
+}
+
+template <typename T, int N1, int NF>
+static int launch_col_t(int ct, int count, const OpoInst *insts, const ColArgs &a, hipStream_t s, hipEvent_t e0,
+                        hipEvent_t e1) {
+  if (a.spt == 4) return launch_col_e<T, N1, NF, 4>(ct, count, insts, a, s, e0, e1);
+  if (a.spt == 8) return launch_col_e<T, N1, NF, 8>(ct, count, insts, a, s, e0, e1);
+  return (int)hipErrorInvalidValue;
+}
+
+template <typename T, int N2, int NF, int E>
+static int launch_row_e(int rt, int count, const OpoInst *insts, const RowArgs &a, hipStream_t s, hipEvent_t e0,
+                        hipEvent_t e1) {
+  using V = typename vec2<T>::type;
+  const int threads = rt * (N2 / E);
+  const size_t lds = 2 * (size_t)rt * (N2 + N2 / 8) * sizeof(V);
+  dim3 grid((a.N1 / rt) * a.nf * (a.xcd_pack ? 8 : 1), count);
+  static std::once_flag big_lds[2][kMaxDevices];
+  auto kern = a.direct ? &row_stage<T, N2, NF, true, E> : &row_stage<T, N2, NF, false, E>;
+  if (lds > 65536) {
+    const int rc = opt_in_big_lds(reinterpret_cast<const void *>(kern), big_lds[a.direct ? 1 : 0]);
     if (rc != 0) return rc;
   }
   if (e0 || e1) hipExtLaunchKernelGGL(kern, grid, dim3(threads), lds, s, e0, e1, 0, insts, a);
@@ -509,20 +544,10 @@ static int launch_col_t(int ct, int count, const OpoInst *insts, const ColArgs &
 template <typename T, int N2, int NF>
 static int launch_row_t(int rt, int count, int tiles_y, const OpoInst *insts, const RowArgs &a, hipStream_t s,
                         hipEvent_t e0, hipEvent_t e1) {
-  using V = typename vec2<T>::type;
-  const int threads = rt * (N2 / 8);
-  const size_t lds = 2 * (size_t)rt * (N2 + N2 / 8) * sizeof(V);
-  dim3 grid((a.N1 / rt) * a.nf * (a.xcd_pack ? 8 : 1), count);
   (void)tiles_y;
-  static std::once_flag big_lds[2][kMaxDevices];
-  auto kern = a.direct ? &row_stage<T, N2, NF, true> : &row_stage<T, N2, NF, false>;
-  if (lds > 65536) {
-    const int rc = opt_in_big_lds(reinterpret_cast<const void *>(kern), big_lds[a.direct ? 1 : 0]);
-    if (rc != 0) return rc;
-  }
-  if (e0 || e1) hipExtLaunchKernelGGL(kern, grid, dim3(threads), lds, s, e0, e1, 0, insts, a);
-  else hipLaunchKernelGGL(kern, grid, dim3(threads), lds, s, insts, a);
-  return (int)hipGetLastError();
+  if (a.spt == 4) return launch_row_e<T, N2, NF, 4>(rt, count, insts, a, s, e0, e1);
+  if (a.spt == 8) return launch_row_e<T, N2, NF, 8>(rt, count, insts, a, s, e0, e1);
+  return (int)hipErrorInvalidValue;
 }
 
 #define OPO_COL_SIZES(X) X(16) X(32) X(64) X(128)

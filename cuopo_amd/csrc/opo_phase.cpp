@@ -14,14 +14,14 @@
 //         + [z (D + b2 alpha_h) v k2 + z b2 v^2 k2^2/2] column term S[k2]
 //         + z b2 s v k1 k2                              cross term  X(k1 k2)
 // and h is fixed by k2 (q >= N/2 <=> k2 >= N2/2).  The row stage's thread t
-// holds k2 = t + m N2/8 (m < 8), so h = (m >= 4) and
-//   X(k1 k2) = X(k1 t) X(k1 m N2/8),
-// which gives per field the tables
-//   Z[8 k1 + m] = R_{m>=4}[k1] X(k1 m N2/8)   (8 N1; uniform per row)
-//   S[k2]                                     (N2)
-//   Xhi[e >> L], Xlo[e & (2^L-1)] = X(e), e = k1 t < N/8   (two short tables)
-// and the phase of element m is Z[8 k1 + m] S[t + m N2/8] X(k1 t): two
-// independent complex products per element, 8N1 + N2 + N/2^(L+3) + 2^L
+// holds k2 = t + m N2/E (m < E, E = 4 or 8 samples per thread), so
+// h = (m >= E/2) and X(k1 k2) = X(k1 t) X(k1 m N2/E), which gives per field
+// the tables
+//   Z[E k1 + m] = R_{m>=E/2}[k1] X(k1 m N2/E)   (E N1; uniform per row)
+//   S[k2]                                       (N2)
+//   Xhi[e >> L], Xlo[e & (2^L-1)] = X(e), e = k1 t < N/E   (two short tables)
+// and the phase of element m is Z[E k1 + m] S[t + m N2/E] X(k1 t): two
+// independent complex products per element, E N1 + N2 + N/(E 2^L) + 2^L
 // values per field instead of N.
 #include <cfloat>
 #include <cmath>
@@ -63,20 +63,21 @@ bool affine_grid(const double *w, int N, double &a0, double &a1, double &s) {
   return true;
 }
 
-size_t lin_factor_len(int N, int N1, int xl_bits) {
-  return 8 * (size_t)N1 + (size_t)(N / N1) + (((size_t)N / 8) >> xl_bits) + ((size_t)1 << xl_bits);
+size_t lin_factor_len(int N, int N1, int xl_bits, int E) {
+  return (size_t)E * N1 + (size_t)(N / N1) + (((size_t)N / E) >> xl_bits) + ((size_t)1 << xl_bits);
 }
 
-int lin_factors(const opo_params &p, const double *w, int N1, int xl_bits, double *out) {
-  const int N = p.N, N2 = N / N1, T2 = N2 / 8;
-  if (N2 < 8 || N2 % 8 != 0 || (size_t)(N / 8) % ((size_t)1 << xl_bits) != 0) return 0;
+int lin_factors(const opo_params &p, const double *w, int N1, int xl_bits, int E, double *out) {
+  const int N = p.N, N2 = N / N1;
+  if ((E != 4 && E != 8) || N2 < E || N2 % E != 0 || (size_t)(N / E) % ((size_t)1 << xl_bits) != 0) return 0;
+  const int T2 = N2 / E;
   double D[3], b2[3], b3[3], al[3];
   const int NF = lin_coeffs(p, D, b2, b3, al);
   for (int f = 0; f < NF; ++f)
     if (b3[f] != 0) return 0;
   double a0, a1, s;
   if (!affine_grid(w, N, a0, a1, s)) return 0;
-  const size_t FB = lin_factor_len(N, N1, xl_bits);
+  const size_t FB = lin_factor_len(N, N1, xl_bits, E);
   const long double z = p.dz, v = (long double)s * N1;
   auto put = [](double *o, long double th, long double amp) {
     o[0] = (double)(amp * cosl(th));
@@ -88,18 +89,18 @@ int lin_factors(const opo_params &p, const double *w, int N1, int xl_bits, doubl
     const long double amp = expl(-0.5L * al[f] * z) / N;
     const long double beta = z * bf * (long double)s * v;  // cross term coefficient
     for (int k1 = 0; k1 < N1; ++k1)
-      for (int m = 0; m < 8; ++m) {
-        const long double u = (m >= 4 ? a1 : a0) + (long double)s * k1;
+      for (int m = 0; m < E; ++m) {
+        const long double u = (2 * m >= E ? a1 : a0) + (long double)s * k1;
         const long double row = z * Df * u + 0.5L * z * bf * u * u;
-        put(o + 2 * ((size_t)8 * k1 + m), row + beta * (long double)k1 * m * T2, amp);
+        put(o + 2 * ((size_t)E * k1 + m), row + beta * (long double)k1 * m * T2, amp);
       }
-    o += 16 * (size_t)N1;
+    o += 2 * (size_t)E * N1;
     for (int k2 = 0; k2 < N2; ++k2) {
       const long double ah = k2 >= N2 / 2 ? a1 : a0;
       put(o + 2 * (size_t)k2, z * (Df + bf * ah) * v * k2 + 0.5L * z * bf * v * v * (long double)k2 * k2, 1.0L);
     }
     o += 2 * (size_t)N2;
-    const size_t nh = ((size_t)N / 8) >> xl_bits, nl = (size_t)1 << xl_bits;
+    const size_t nh = ((size_t)N / E) >> xl_bits, nl = (size_t)1 << xl_bits;
     for (size_t j = 0; j < nh; ++j) put(o + 2 * j, beta * (long double)(j << xl_bits), 1.0L);
     o += 2 * nh;
     for (size_t l = 0; l < nl; ++l) put(o + 2 * l, beta * (long double)l, 1.0L);
@@ -109,7 +110,8 @@ int lin_factors(const opo_params &p, const double *w, int N1, int xl_bits, doubl
 
 }  // namespace opo
 
-extern "C" int opo_linear_factors(const opo_params *p, const double *w, int N1, int xl_bits, double *out) {
-  if (!p || !w || !out || N1 < 2 || p->N % N1 != 0 || xl_bits < 0 || xl_bits > 30) return OPO_ERR_INVALID;
-  return opo::lin_factors(*p, w, N1, xl_bits, out);
+extern "C" int opo_linear_factors(const opo_params *p, const double *w, int N1, int E, int xl_bits, double *out) {
+  if (!p || !w || !out || N1 < 2 || p->N % N1 != 0 || (E != 4 && E != 8) || xl_bits < 0 || xl_bits > 30)
+    return OPO_ERR_INVALID;
+  return opo::lin_factors(*p, w, N1, xl_bits, E, out);
 }

@@ -109,15 +109,16 @@ void opo_noise(uint64_t seed, uint64_t stream, double sigma, double *A, size_t n
 double opo_find_pm_temp(int crystal, double lp, double ls, double Lambda, double Tmin, double Tmax,
                         int pm_mode);
 /* Factorised phase of the linear step (cwes3.cuh:112-125 / cwes2.cuh:108-118)
- * that the row stage uses on the reference's frequency grid.  Per field:
- * Z (8*N1), S (N/N1), Xhi ((N/8) >> xl_bits), Xlo (1 << xl_bits) complex
- * values with, for frequency q = k1 + N1 k2, k2 = t + m N/(8 N1) (m < 8),
- * e = k1 t:  e^{i phase_q} e^{-alpha dz/2}/N =
- * Z[8 k1 + m] S[k2] Xhi[e >> xl_bits] Xlo[e & (2^xl_bits - 1)].
+ * that the row stage uses on the reference's frequency grid, for a row FFT of
+ * E = 4 or 8 samples per thread.  Per field: Z (E*N1), S (N/N1),
+ * Xhi ((N/E) >> xl_bits), Xlo (1 << xl_bits) complex values with, for
+ * frequency q = k1 + N1 k2, k2 = t + m N/(E N1) (m < E), e = k1 t:
+ * e^{i phase_q} e^{-alpha dz/2}/N =
+ * Z[E k1 + m] S[k2] Xhi[e >> xl_bits] Xlo[e & (2^xl_bits - 1)].
  * Returns 1 (out filled), 0 (w is not the affine two-half grid or a
  * third-order term is nonzero: the solver then uses the full phase table) or
  * OPO_ERR_INVALID.  No reference counterpart. */
-int opo_linear_factors(const opo_params *p, const double *w, int N1, int xl_bits, double *out);
+int opo_linear_factors(const opo_params *p, const double *w, int N1, int E, int xl_bits, double *out);
 /* Refractive index and dispersion of the crystal (ppln.cuh / spplt.cuh). */
 double opo_n(int crystal, double L, double T);
 double opo_group_velocity(int crystal, double L, double T);
@@ -155,13 +156,15 @@ int opo_set_tiles(opo_solver *s, int col_threads, int row_threads);
 /* Launch settings of the instances of one grid size (no reference
  * counterpart).  Inputs: col_threads / row_threads (threads per field of a
  * column tile / per row tile; 0 = default), xcd_pack (-1 default, 0, 1),
- * phase_mode (-1 auto, 0 full phase table, 1 factorised tables).  Outputs of
- * get: N, count, col_tile / row_tile (columns / rows per block), factored
- * (the factorised phase is in use) and slice_us (col + row stage time of the
- * autotuner's choice, 0 if not tuned). */
+ * phase_mode (-1 auto, 0 full phase table, 1 factorised tables),
+ * col_spt / row_spt (samples per thread of the column / row FFTs: 8 =
+ * radix-8 passes, 4 = radix-4 passes with twice the threads; 0 = 8).
+ * Outputs of get: N, count, col_tile / row_tile (columns / rows per block),
+ * factored (the factorised phase is in use) and slice_us (col + row stage
+ * time of the autotuner's choice, 0 if not tuned). */
 typedef struct {
   int N, count;
-  int col_threads, row_threads, xcd_pack, phase_mode;
+  int col_threads, row_threads, xcd_pack, phase_mode, col_spt, row_spt;
   int col_tile, row_tile, factored;
   double slice_us;
 } opo_tuning;

@@ -46,12 +46,14 @@ def test_every_setting_gives_the_same_round_trip(gpu):
     P = gpu
     cfg = c2(N=16384, NZ=6, NRT=3)
     ref = None
-    for col, row, xcd, ph in [(0, 0, 0, 0), (0, 0, 0, 1), (64, 256, 0, 1), (128, 512, 1, 0), (32, 64, 1, 1),
-                              (32, 128, 0, 0)]:
+    for col, row, xcd, ph, ce, re in [(0, 0, 0, 0, 8, 8), (0, 0, 0, 1, 8, 8), (64, 256, 0, 1, 8, 8),
+                                      (128, 512, 1, 0, 8, 8), (32, 64, 1, 1, 8, 8), (32, 128, 0, 0, 8, 8),
+                                      (0, 0, 0, 0, 4, 4), (128, 128, 0, 1, 4, 4), (256, 512, 1, 1, 4, 8),
+                                      (64, 64, 0, 1, 8, 4)]:
         s, _ = solver_for(P, cfg)
-        s.set_tuning(col_threads=col, row_threads=row, xcd_pack=xcd, phase_mode=ph)
+        s.set_tuning(col_threads=col, row_threads=row, xcd_pack=xcd, phase_mode=ph, col_spt=ce, row_spt=re)
         t = s.get_tuning()
-        assert t["factored"] == ph and t["xcd_pack"] == xcd
+        assert t["factored"] == ph and t["xcd_pack"] == xcd and t["col_spt"] == ce and t["row_spt"] == re
         s.run(0, 2)
         out = s.download()
         s.close()
@@ -59,7 +61,7 @@ def test_every_setting_gives_the_same_round_trip(gpu):
             ref = out
             continue
         for a, b in zip(out, ref):
-            assert rel_l2(a, b) < 1e-13, (col, row, xcd, ph)
+            assert rel_l2(a, b) < 1e-13, (col, row, xcd, ph, ce, re)
 
 
 def test_batch_autotune_mixed_sizes(gpu):
@@ -99,3 +101,10 @@ def test_set_tuning_rejects_bad_values(gpu):
         s.set_tuning(col_threads=48)
     with pytest.raises(P.OpoError):
         s.set_tuning(phase_mode=2)
+    with pytest.raises(P.OpoError):
+        s.set_tuning(col_spt=2)
+    # a rejected setting leaves the previous one in place
+    s.set_tuning(row_threads=128, row_spt=4)
+    with pytest.raises(P.OpoError):
+        s.set_tuning(row_threads=128, row_spt=16)
+    assert s.get_tuning()["row_spt"] == 4

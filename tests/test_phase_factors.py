@@ -27,28 +27,28 @@ def direct_phase(p, w):
             for D, b2, b3, al in coef]
 
 
-def factors(p, w, N1, xl):
+def factors(p, w, N1, xl, E=8):
     N = p.N
     nf = 3 if p.three_eqs else 2
-    FB = 8 * N1 + N // N1 + ((N // 8) >> xl) + (1 << xl)
+    FB = E * N1 + N // N1 + ((N // E) >> xl) + (1 << xl)
     out = np.zeros(2 * nf * FB)
-    rc = P.lib().opo_linear_factors(C.byref(p), w.ctypes.data, N1, xl, out.ctypes.data)
+    rc = P.lib().opo_linear_factors(C.byref(p), w.ctypes.data, N1, E, xl, out.ctypes.data)
     return rc, out.view(np.complex128).reshape(nf, FB)
 
 
-def rebuild(tab, N, N1, xl):
+def rebuild(tab, N, N1, xl, E=8):
     """Phase of natural frequency q = k1 + N1 k2 from the tables: the row
-    stage's thread t holds k2 = t + m N2/8."""
+    stage's thread t holds k2 = t + m N2/E."""
     N2 = N // N1
-    T2 = N2 // 8
+    T2 = N2 // E
     q = np.arange(N)
     k1, k2 = q % N1, q // N1
     t, m = k2 % T2, k2 // T2
     e = k1 * t
-    Z, S = tab[: 8 * N1], tab[8 * N1: 8 * N1 + N2]
-    XH = tab[8 * N1 + N2: 8 * N1 + N2 + ((N // 8) >> xl)]
-    XL = tab[8 * N1 + N2 + ((N // 8) >> xl):]
-    return Z[8 * k1 + m] * S[k2] * XH[e >> xl] * XL[e & ((1 << xl) - 1)]
+    Z, S = tab[: E * N1], tab[E * N1: E * N1 + N2]
+    XH = tab[E * N1 + N2: E * N1 + N2 + ((N // E) >> xl)]
+    XL = tab[E * N1 + N2 + ((N // E) >> xl):]
+    return Z[E * k1 + m] * S[k2] * XH[e >> xl] * XL[e & ((1 << xl) - 1)]
 
 
 def product_setup(args):
@@ -76,13 +76,14 @@ def test_factorised_phase_matches_direct(case):
     for N1 in sorted({16, 64, 128, min(1 << (L // 2), 64)}):
         if N // N1 < 16:
             continue
-        for xl in {(L - 2) // 2, (L - 4) // 2}:
-            rc, tab = factors(p, w, N1, xl)
-            assert rc == 1, (case, N1, xl)
-            for f, r in enumerate(ref):
-                got = rebuild(tab[f], N, N1, xl)
-                err = np.max(np.abs(got / r - 1))
-                assert err < 2e-13, (case, N1, xl, f, err)
+        for E in (4, 8):
+            for xl in {(L - 2) // 2, (L - 4) // 2}:
+                rc, tab = factors(p, w, N1, xl, E)
+                assert rc == 1, (case, N1, xl, E)
+                for f, r in enumerate(ref):
+                    got = rebuild(tab[f], N, N1, xl, E)
+                    err = np.max(np.abs(got / r - 1))
+                    assert err < 2e-13, (case, N1, xl, E, f, err)
 
 
 def test_factorised_phase_is_not_trivial():
@@ -107,4 +108,5 @@ def test_generic_grid_and_third_order_fall_back():
     assert factors(p, w2, 64, 6)[0] == 0
     p.b3s = 1e-6
     assert factors(p, w, 64, 6)[0] == 0
-    assert P.lib().opo_linear_factors(None, w.ctypes.data, 64, 6, w.ctypes.data) == P.ERR_INVALID
+    assert P.lib().opo_linear_factors(None, w.ctypes.data, 64, 8, 6, w.ctypes.data) == P.ERR_INVALID
+    assert P.lib().opo_linear_factors(C.byref(p), w.ctypes.data, 64, 5, 6, w.ctypes.data) == P.ERR_INVALID
