@@ -312,6 +312,10 @@ def main():
                      "algorithmic_bytes_per_launch": Bz},
         "extra": {"device_ms_timed_region": dev_ms, "col_stage_us": col_us, "row_stage_us": row_us,
                   "slice_frac_of_hbm": Bz / ((col_us + row_us) * 1e-6) / HBM_PEAK,
+                  # one slice inside the graph replay of the timed region (both
+                  # stages back to back, plus the round trip's boundary and tick)
+                  "graph_slice_us": 1e3 * dev_ms / (a.steps * args.NZ),
+                  "graph_slice_frac_of_hbm": 2 * Bz / (1e-3 * dev_ms / (a.steps * args.NZ)) / HBM_PEAK,
                   "copy_probe_GBps": copy_gbps, "tuning": tuning,
                   "intracavity_power_end": float(np.sum(np.abs(As) ** 2))},
     }
