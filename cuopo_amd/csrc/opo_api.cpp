@@ -968,6 +968,9 @@ int opo_power_history(opo_solver *s, double *out, int n) {
 }
 
 // average kernel time of the slice-interior column stage and the LIN row stage
+// Average device time of one column-stage and one row-stage launch, each
+// timed alone between events on the group's stream (what a kernel profiler
+// reports per dispatch).
 int engine_time_stages(Engine &E, Group &g, int iters, double *col_us, double *row_us) {
   using namespace opo;
   for (auto &I : g.inst)
@@ -996,6 +999,39 @@ int engine_time_stages(Engine &E, Group &g, int iters, double *col_us, double *r
   if (rc != OPO_OK) return rc;
   if (col_us) *col_us = 1e3 * tc / iters;
   if (row_us) *row_us = 1e3 * tr / iters;
+  return OPO_OK;
+}
+
+// Average device time of one crystal slice as a round trip runs it: after a
+// warm-up slice, iters back-to-back (column stage, row stage) pairs between
+// two events on the group's stream.  The autotuner's objective.
+int engine_time_slices(Engine &E, Group &g, int iters, double *slice_us) {
+  using namespace opo;
+  for (auto &I : g.inst)
+    if (!I.uploaded) return fail(OPO_ERR_STATE, "time_slices before upload");
+  RCHK(sync_insts(E));
+  RCHK(reset_counters(E, 0));
+  const int A = all_mask(g);
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int rc = OPO_OK;
+  for (auto &e : ev)
+    if (rc == OPO_OK && hipEventCreate(&e) != hipSuccess) rc = fail(OPO_ERR_HIP, "event create");
+  auto slice = [&] {
+    const int r = emit_col(E, g, OP_NL_END | OP_NL_START, A, A, A, A, 0, 1 % E.NZ, 0, 0);
+    return r != OPO_OK ? r : emit_row(E, g, false, A);
+  };
+  if (rc == OPO_OK) rc = slice();
+  if (rc == OPO_OK && hipEventRecord(ev[0], g.cur) != hipSuccess) rc = fail(OPO_ERR_HIP, "event record");
+  for (int it = 0; it < iters && rc == OPO_OK; ++it) rc = slice();
+  if (rc == OPO_OK && hipEventRecord(ev[1], g.cur) != hipSuccess) rc = fail(OPO_ERR_HIP, "event record");
+  float ms = 0;
+  if (rc == OPO_OK && (hipEventSynchronize(ev[1]) != hipSuccess || hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess))
+    rc = fail(OPO_ERR_HIP, "event");
+  if (rc != OPO_OK) (void)hipStreamSynchronize(g.cur);
+  for (auto &e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (rc != OPO_OK) return rc;
+  *slice_us = 1e3 * ms / iters;
   return OPO_OK;
 }
 
@@ -1220,12 +1256,14 @@ int set_tuning(Engine &E, Group &g, const opo_tuning *t) {
   return OPO_OK;
 }
 
-// Time the candidates of each group with engine_time_stages and keep the
-// fastest: for each XCD packing, the column settings (tile of 1/2, 1 or 2
-// cache lines x 4 or 8 samples per thread) are timed with the row settings
-// fixed, then the row settings (64..512-thread tiles x 4 or 8 samples per
-// thread x phase source) with the best column settings.  The stage runs
-// mutate the fields, so each group's state is saved before and restored after.
+// Time the candidates of each group with engine_time_slices (back-to-back
+// slices, as a round trip runs them) and keep the fastest: for each XCD
+// packing, the column settings (tile of 1/2, 1 or 2 cache lines, and 1/4 for
+// small groups, x 4 or 8 samples per thread) with the row settings fixed,
+// then the row settings (64..512-thread tiles, and 32 for small groups, x 4
+// or 8 samples per thread x phase source) with the best column settings.
+// The timed slices mutate the fields, so each group's state is saved before
+// and restored after.
 int engine_autotune(Engine &E, int iters) {
   for (auto &g : E.groups) {
     for (auto &I : g.inst)
@@ -1243,15 +1281,20 @@ int engine_autotune(Engine &E, int iters) {
     const int ct0 = (int)(128 / esize(g));
     std::vector<int> xcds = {0};
     if ((long long)n * g.N <= (1LL << 16)) xcds.push_back(1);
+    // latency-bound groups (one launch is a single wave of blocks) also try
+    // quarter-line column tiles and one-row row tiles: more, smaller blocks
+    const bool small = (long long)n * g.N <= (1LL << 17);
+    std::vector<int> cts = {ct0 / 2, ct0, ct0 * 2}, rts = {64, 128, 256, 512};
+    if (small) { cts.insert(cts.begin(), ct0 / 4); rts.insert(rts.begin(), 32); }
     bool can_fac = true;
     for (auto &I : g.inst) can_fac &= I.use_fac;
     std::vector<int> phases = {0};
     if (can_fac) phases.push_back(1);
-    auto time_it = [&](const opo_tuning &t, double *cu, double *ru) -> int {
+    auto time_it = [&](const opo_tuning &t, double *us) -> int {
       opo_tuning tt = t;
       if (set_tuning(E, g, &tt) != OPO_OK) return 1;  // shape does not fit: skip
       RCHK(sync_insts(E));
-      RCHK(engine_time_stages(E, g, iters, cu, ru));
+      RCHK(engine_time_slices(E, g, iters, us));
       return OPO_OK;
     };
     opo_tuning best = base;
@@ -1263,29 +1306,29 @@ int engine_autotune(Engine &E, int iters) {
       double bc = 1e30;
       opo_tuning tc = t;
       for (int spt : {8, 4})
-        for (int ct : {ct0 / 2, ct0, ct0 * 2}) {
+        for (int ct : cts) {
           const int th = ct * (g.N1 / spt);
           if (th < 32 || th > 512 || (th & (th - 1))) continue;
           opo_tuning c = t;
           c.col_threads = th; c.col_spt = spt;
-          double cu = 0, ru = 0;
-          const int rc = time_it(c, &cu, &ru);
+          double us = 0;
+          const int rc = time_it(c, &us);
           if (rc < 0) return rc;
-          if (rc == OPO_OK && cu < bc) { bc = cu; tc = c; }
+          if (rc == OPO_OK && us < bc) { bc = us; tc = c; }
         }
       double br = 1e30;
       opo_tuning tr = tc;
       for (int spt : {8, 4})
-        for (int r : {64, 128, 256, 512})
+        for (int r : rts)
           for (int ph : phases) {
             opo_tuning c = tc;
             c.row_threads = r; c.row_spt = spt; c.phase_mode = ph;
-            double cu = 0, ru = 0;
-            const int rc = time_it(c, &cu, &ru);
+            double us = 0;
+            const int rc = time_it(c, &us);
             if (rc < 0) return rc;
-            if (rc == OPO_OK && ru < br) { br = ru; tr = c; }
+            if (rc == OPO_OK && us < br) { br = us; tr = c; }
           }
-      if (bc + br < best_us) { best_us = bc + br; best = tr; }
+      if (br < best_us) { best_us = br; best = tr; }
     }
     RCHK(set_tuning(E, g, &best));
     g.tuned_us = best_us;

@@ -160,17 +160,17 @@ int opo_set_tiles(opo_solver *s, int col_threads, int row_threads);
  * col_spt / row_spt (samples per thread of the column / row FFTs: 8 =
  * radix-8 passes, 4 = radix-4 passes with twice the threads; 0 = 8).
  * Outputs of get: N, count, col_tile / row_tile (columns / rows per block),
- * factored (the factorised phase is in use) and slice_us (col + row stage
- * time of the autotuner's choice, 0 if not tuned). */
+ * factored (the factorised phase is in use) and slice_us (time of one slice,
+ * both stages back to back, of the autotuner's choice; 0 if not tuned). */
 typedef struct {
   int N, count;
   int col_threads, row_threads, xcd_pack, phase_mode, col_spt, row_spt;
   int col_tile, row_tile, factored;
   double slice_us;
 } opo_tuning;
-/* Times the candidate settings on the uploaded instances (iters launches of
- * each stage per candidate) and keeps the fastest; the fields are restored
- * and the round-trip counters reset afterwards. */
+/* Times the candidate settings on the uploaded instances (iters back-to-back
+ * slices per candidate) and keeps the fastest; the fields are restored and
+ * the round-trip counters reset afterwards. */
 int opo_autotune(opo_solver *s, int iters);
 int opo_get_tuning(opo_solver *s, opo_tuning *out);
 int opo_set_tuning(opo_solver *s, const opo_tuning *t);

@@ -49,7 +49,7 @@ def test_every_setting_gives_the_same_round_trip(gpu):
     for col, row, xcd, ph, ce, re in [(0, 0, 0, 0, 8, 8), (0, 0, 0, 1, 8, 8), (64, 256, 0, 1, 8, 8),
                                       (128, 512, 1, 0, 8, 8), (32, 64, 1, 1, 8, 8), (32, 128, 0, 0, 8, 8),
                                       (0, 0, 0, 0, 4, 4), (128, 128, 0, 1, 4, 4), (256, 512, 1, 1, 4, 8),
-                                      (64, 64, 0, 1, 8, 4)]:
+                                      (64, 64, 0, 1, 8, 4), (32, 32, 0, 0, 4, 8), (32, 32, 0, 1, 4, 4)]:
         s, _ = solver_for(P, cfg)
         s.set_tuning(col_threads=col, row_threads=row, xcd_pack=xcd, phase_mode=ph, col_spt=ce, row_spt=re)
         t = s.get_tuning()
