@@ -153,6 +153,38 @@ __global__ __launch_bounds__(E == 4 ? 1024 : 512) void col_stage(const OpoInst *
   const bool any_fft = (inv_mask | fwd_mask) != 0;
 
   const unsigned ops = MID ? (OP_NL_END | OP_NL_START) : a.ops;
+  // Issue order of the loads: the field samples first, then the four-step
+  // twiddle gathers, then the column root table and the NL coefficients.  A
+  // wait for any of the later ones (the table's LDS copy waits for its load;
+  // a scalar-load wait covers every scalar load in flight) then no longer
+  // holds back the field loads, so the kernel pays one memory latency.
+  V v[E];
+  const bool ld = (load_mask >> f) & 1;
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = ld ? st[f * N + (long long)N2 * (t + m * T1) + n2] : mk<V>(0, 0);
+  // four-step twiddles W_m = e^{2 pi i n2 k1 / N} for this thread's
+  // k1 = t + m*T1, from two-level root tables: X(e) = hi[e >> lobits] lo[e & mask]
+  const V *twlo = reinterpret_cast<const V *>(a.twlo);
+  const V *twhi = reinterpret_cast<const V *>(a.twhi);
+  const unsigned lm = (1u << a.lobits) - 1u;
+  constexpr bool F64 = std::is_same<T, double>::value;
+  // gathered roots: FP64 x0 and the base b1 (hi, lo each); FP32 one pair per m
+  V gh[F64 ? 1 : E], gl[F64 ? 1 : E];
+  V x0h{}, x0l{}, b1h{}, b1l{};
+  if (any_fft) {
+    if constexpr (F64) {
+      const unsigned e0 = (unsigned)(n2 * t) & nmask, e1 = (unsigned)(n2 * T1) & nmask;
+      x0h = twhi[e0 >> a.lobits]; x0l = twlo[e0 & lm];
+      b1h = twhi[e1 >> a.lobits]; b1l = twlo[e1 & lm];
+    } else {
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const unsigned e = (unsigned)(n2 * (t + m * T1)) & nmask;
+        gh[m] = twhi[e >> a.lobits]; gl[m] = twlo[e & lm];
+      }
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
   NlCoef<NF, V> nle{}, nls{};
   if constexpr (DIRECT) {
     nle = load_nl<NF, V>(a.d_nl, a.d_kp, a.d_ks, a.d_h, a.s_end);
@@ -161,28 +193,18 @@ __global__ __launch_bounds__(E == 4 ? 1024 : 512) void col_stage(const OpoInst *
     if (ops & OP_NL_END) nle = load_nl<NF, V>(I->nl, I->kp, I->ks, I->h, a.s_end);
     if (ops & OP_NL_START) nls = load_nl<NF, V>(I->nl, I->kp, I->ks, I->h, a.s_start);
   }
-  if (any_fft) {
-    const V *tw1 = reinterpret_cast<const V *>(a.tw1);
-    for (int e = tid; e < N1; e += blockDim.x) ltw[e] = tw1[e];
-  }
-  V v[E];
-  const bool ld = (load_mask >> f) & 1;
-#pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = ld ? st[f * N + (long long)N2 * (t + m * T1) + n2] : mk<V>(0, 0);
-  // four-step twiddles W_m = e^{2 pi i n2 k1 / N} for this thread's
-  // k1 = t + m*T1, from two-level root tables: X(e) = hi[e >> lobits] lo[e & mask]
   V W[E];
   if (any_fft) {
-    const V *twlo = reinterpret_cast<const V *>(a.twlo);
-    const V *twhi = reinterpret_cast<const V *>(a.twhi);
-    const unsigned lm = (1u << a.lobits) - 1u;
-    if constexpr (std::is_same<T, double>::value) {
+    // column root table into LDS (the block has NF*FT threads; blockDim
+    // would cost a scalar load whose wait covers the NL loads)
+    const V *tw1 = reinterpret_cast<const V *>(a.tw1);
+    for (int e = tid; e < N1; e += NF * FT) ltw[e] = tw1[e];
+    if constexpr (F64) {
       // W_m = X(n2 t) X(n2 T1)^m, the powers from b, b^2, b^4 by the bits of m
       // (at most three complex products per twiddle; the base's rounding
       // grows at most 7x, ~1e-15 in FP64): 4 gathers per thread instead of 2E
-      const unsigned e0 = (unsigned)(n2 * t) & nmask, e1 = (unsigned)(n2 * T1) & nmask;
-      const V x0 = cmul(twhi[e0 >> a.lobits], twlo[e0 & lm]);
-      const V b1 = cmul(twhi[e1 >> a.lobits], twlo[e1 & lm]);
+      const V x0 = cmul(x0h, x0l);
+      const V b1 = cmul(b1h, b1l);
       const V b2 = cmul(b1, b1), b4 = cmul(b2, b2);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
@@ -195,10 +217,7 @@ __global__ __launch_bounds__(E == 4 ? 1024 : 512) void col_stage(const OpoInst *
     } else {
       // FP32: the same growth would cost accuracy, so each twiddle is looked up
 #pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const unsigned e = (unsigned)(n2 * (t + m * T1)) & nmask;
-        W[m] = cmul(twhi[e >> a.lobits], twlo[e & lm]);
-      }
+      for (int m = 0; m < E; ++m) W[m] = cmul(gh[m], gl[m]);
     }
   }
 
