@@ -91,6 +91,12 @@ template <int R, int S, typename V> __device__ __forceinline__ void dftr(V* u) {
 // thread) skips the arithmetic of threads whose data is not transformed while
 // every thread still reaches the barriers.
 //
+// `wave` (block-uniform): the team is one wavefront or part of one, and no
+// other team shares its LDS region.  LDS operations of a wavefront execute in
+// issue order, so the team then exchanges through a single buffer in place
+// (every lane's reads of an exchange are issued before any lane's writes of
+// the next) with wave-scope ordering only: half the LDS and no block barrier.
+//
 // Twiddles e^{+2 pi i e/M} come from `tws`: with TREG false an M-point root
 // table (LDS) indexed by e; with TREG true this thread's register set filled
 // by load_twiddles, indexed by (pass, butterfly, leg).
@@ -103,16 +109,25 @@ constexpr int tw_off(int M, int Q, int E) {
 }
 constexpr int tw_count(int M, int E) { return tw_off(M, npasses(M, E), E); }
 
+// orders this wavefront's LDS accesses (no instruction: a wave's LDS
+// operations already execute in order; the fences keep the compiler from
+// moving them across)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int M, int Q, int S, bool PAD, bool TREG, int E, typename V>
 __device__ __forceinline__ void fft_pass(V *v, bool on, V *buf0, int &ph, int t, int gbase, int es, int bufstride,
-                                         const V *tws) {
+                                         const V *tws, bool wave) {
   constexpr int R = radix_at(M, Q, E);
   constexpr int P = p_at(Q, E);
   constexpr int T = M / E;
   constexpr int NB = E / R;
   constexpr bool LAST = Q == npasses(M, E) - 1;
   constexpr int TOFF = tw_off(M, Q, E);  // constant: keeps the register set statically indexed
-  V *buf = buf0 + (ph & 1) * bufstride;
+  V *buf = wave ? buf0 : buf0 + (ph & 1) * bufstride;
   if (on) {
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
@@ -140,21 +155,23 @@ __device__ __forceinline__ void fft_pass(V *v, bool on, V *buf0, int &ph, int t,
     }
   }
   if constexpr (!LAST) {
-    __syncthreads();
+    if (wave) wave_sync();
+    else __syncthreads();
     if (on) {
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = buf[gbase + lidx<PAD>(t + m * T, es)];
     }
+    if (wave) wave_sync();
     ++ph;
   }
 }
 
 template <int M, int S, bool PAD, bool TREG, int E, int Q = 0, typename V>
 __device__ __forceinline__ void team_fft(V *v, bool on, V *buf0, int &ph, int t, int gbase, int es, int bufstride,
-                                         const V *tws) {
+                                         const V *tws, bool wave) {
   if constexpr (Q < npasses(M, E)) {
-    fft_pass<M, Q, S, PAD, TREG, E>(v, on, buf0, ph, t, gbase, es, bufstride, tws);
-    team_fft<M, S, PAD, TREG, E, Q + 1>(v, on, buf0, ph, t, gbase, es, bufstride, tws);
+    fft_pass<M, Q, S, PAD, TREG, E>(v, on, buf0, ph, t, gbase, es, bufstride, tws, wave);
+    team_fft<M, S, PAD, TREG, E, Q + 1>(v, on, buf0, ph, t, gbase, es, bufstride, tws, wave);
   }
 }
 

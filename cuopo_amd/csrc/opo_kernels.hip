@@ -117,7 +117,7 @@ __global__ __launch_bounds__(E == 4 ? 1024 : 512) void col_stage(const OpoInst *
   constexpr int T1 = N1 / E;
   extern __shared__ unsigned char smem_raw[];
   V *ltw = reinterpret_cast<V *>(smem_raw);  // N1-point root table
-  V *lds = ltw + N1;                         // 2 buffers x NF fields x CT*N1
+  V *lds = ltw + N1;                         // 2 buffers (1 for wave) x NF fields x CT*N1
 
   const OpoInst *I = insts + blockIdx.y;
   if constexpr (!DIRECT) {
@@ -134,6 +134,9 @@ __global__ __launch_bounds__(E == 4 ? 1024 : 512) void col_stage(const OpoInst *
 
   const int CT = 1 << a.log2ct;
   const int FT = CT * T1;
+  // slice interior with one wavefront (or less) per field: single-buffer
+  // wave-private column exchanges (the launcher sizes the LDS to match)
+  const bool wave = MID && FT <= 64;
   const int tid = threadIdx.x;
   const int f = tid / FT;  // this thread's field
   const int lt = tid - f * FT;
@@ -199,6 +202,7 @@ __global__ __launch_bounds__(E == 4 ? 1024 : 512) void col_stage(const OpoInst *
     // would cost a scalar load whose wait covers the NL loads)
     const V *tw1 = reinterpret_cast<const V *>(a.tw1);
     for (int e = tid; e < N1; e += NF * FT) ltw[e] = tw1[e];
+    if (wave) __syncthreads();  // the exchanges no longer order the table's writes
     if constexpr (F64) {
       // W_m = X(n2 t) X(n2 T1)^m, the powers from b, b^2, b^4 by the bits of m
       // (at most three complex products per twiddle; the base's rounding
@@ -230,10 +234,10 @@ __global__ __launch_bounds__(E == 4 ? 1024 : 512) void col_stage(const OpoInst *
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = cmul(v[m], cconj(W[m]));
     }
-    team_fft<N1, -1, false, false, E>(v, inv, mine0, ph, t, c, CT, bufstride, ltw);
+    team_fft<N1, -1, false, false, E>(v, inv, mine0, ph, t, c, CT, bufstride, ltw, wave);
   }
   // ---- all fields meet in the next LDS buffer ----
-  V *xb = lds + (ph & 1) * bufstride;
+  V *xb = lds + (wave ? 0 : (ph & 1) * bufstride);
 #pragma unroll
   for (int m = 0; m < E; ++m) xb[f * fstride + c + (t + m * T1) * CT] = v[m];
   __syncthreads();
@@ -362,7 +366,7 @@ __global__ __launch_bounds__(E == 4 ? 1024 : 512) void col_stage(const OpoInst *
   // ---- forward column transform of this group's field ----
   const bool fw = (fwd_mask >> f) & 1;
   if (fwd_mask) {
-    team_fft<N1, +1, false, false, E>(v, fw, mine0, ph, t, c, CT, bufstride, ltw);
+    team_fft<N1, +1, false, false, E>(v, fw, mine0, ph, t, c, CT, bufstride, ltw, wave);
     if (fw) {
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = cmul(v[m], W[m]);
@@ -384,7 +388,7 @@ __global__ __launch_bounds__(row_max_threads(N2, E)) void row_stage(const OpoIns
   constexpr int T2 = N2 / E;
   constexpr int NP2 = N2 + N2 / 8;  // padded row in LDS
   extern __shared__ unsigned char smem_raw[];
-  V *lds = reinterpret_cast<V *>(smem_raw);  // 2 buffers x RT padded rows
+  V *lds = reinterpret_cast<V *>(smem_raw);  // 2 buffers (1 when T2 <= 64) x RT padded rows
 
   const OpoInst *I = insts + blockIdx.y;
   if constexpr (!DIRECT) {
@@ -438,10 +442,11 @@ __global__ __launch_bounds__(row_max_threads(N2, E)) void row_stage(const OpoIns
   }
   int ph = 0;
   const int bufstride = RT * NP2;
-  team_fft<N2, +1, true, true, E>(v, true, lds, ph, t, r * NP2, 1, bufstride, treg);
+  constexpr bool wave = T2 <= 64;  // one row per wavefront or less: see fft_pass
+  team_fft<N2, +1, true, true, E>(v, true, lds, ph, t, r * NP2, 1, bufstride, treg, wave);
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = cmul(v[m], ph_tab[m]);
-  team_fft<N2, -1, true, true, E>(v, true, lds, ph, t, r * NP2, 1, bufstride, treg);
+  team_fft<N2, -1, true, true, E>(v, true, lds, ph, t, r * NP2, 1, bufstride, treg, wave);
 #pragma unroll
   for (int m = 0; m < E; ++m) row[t + m * T2] = v[m];
 }
@@ -517,7 +522,10 @@ static int launch_col_e(int ct, int count, const OpoInst *insts, const ColArgs &
   using V = typename vec2<T>::type;
   dim3 grid((a.N2 / ct) * (a.xcd_pack ? 8 : 1), count);
   const bool mid = col_is_mid<NF>(a);
-  const size_t lds = ((size_t)N1 + 2 * (size_t)NF * ct * N1) * sizeof(V) + 64 * sizeof(double);
+  // one exchange buffer when the slice-interior kernel's field teams are
+  // single wavefronts (the rule col_stage applies), two otherwise
+  const int nbuf = (mid && ct * (N1 / E) <= 64) ? 1 : 2;
+  const size_t lds = ((size_t)N1 + nbuf * (size_t)NF * ct * N1) * sizeof(V) + 64 * sizeof(double);
   const bool direct = mid && a.direct;
   static std::once_flag big_lds[3][kMaxDevices];
   const int kv = direct ? 2 : (mid ? 1 : 0);
@@ -547,7 +555,7 @@ static int launch_row_e(int rt, int count, const OpoInst *insts, const RowArgs &
                         hipEvent_t e1) {
   using V = typename vec2<T>::type;
   const int threads = rt * (N2 / E);
-  const size_t lds = 2 * (size_t)rt * (N2 + N2 / 8) * sizeof(V);
+  const size_t lds = (N2 / E <= 64 ? 1 : 2) * (size_t)rt * (N2 + N2 / 8) * sizeof(V);  // see row_stage
   dim3 grid((a.N1 / rt) * a.nf * (a.xcd_pack ? 8 : 1), count);
   static std::once_flag big_lds[2][kMaxDevices];
   auto kern = a.direct ? &row_stage<T, N2, NF, true, E> : &row_stage<T, N2, NF, false, E>;
