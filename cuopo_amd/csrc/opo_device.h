@@ -109,6 +109,11 @@ constexpr int tw_off(int M, int Q, int E) {
 }
 constexpr int tw_count(int M, int E) { return tw_off(M, npasses(M, E), E); }
 
+// The single-wavefront rule, shared by the kernels and by the launchers that
+// size their LDS (one exchange buffer when it holds, two otherwise): a team
+// of `team` threads (a power of two) fits in one wavefront.
+__host__ __device__ constexpr bool team_in_wave(int team) { return team <= 64; }
+
 // orders this wavefront's LDS accesses (no instruction: a wave's LDS
 // operations already execute in order; the fences keep the compiler from
 // moving them across)

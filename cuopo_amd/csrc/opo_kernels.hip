@@ -136,7 +136,7 @@ __global__ __launch_bounds__(E == 4 ? 1024 : 512) void col_stage(const OpoInst *
   const int FT = CT * T1;
   // slice interior with one wavefront (or less) per field: single-buffer
   // wave-private column exchanges (the launcher sizes the LDS to match)
-  const bool wave = MID && FT <= 64;
+  const bool wave = MID && team_in_wave(FT);
   const int tid = threadIdx.x;
   const int f = tid / FT;  // this thread's field
   const int lt = tid - f * FT;
@@ -442,7 +442,7 @@ __global__ __launch_bounds__(row_max_threads(N2, E)) void row_stage(const OpoIns
   }
   int ph = 0;
   const int bufstride = RT * NP2;
-  constexpr bool wave = T2 <= 64;  // one row per wavefront or less: see fft_pass
+  constexpr bool wave = team_in_wave(T2);  // one row per wavefront or less: see fft_pass
   team_fft<N2, +1, true, true, E>(v, true, lds, ph, t, r * NP2, 1, bufstride, treg, wave);
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = cmul(v[m], ph_tab[m]);
@@ -524,7 +524,7 @@ static int launch_col_e(int ct, int count, const OpoInst *insts, const ColArgs &
   const bool mid = col_is_mid<NF>(a);
   // one exchange buffer when the slice-interior kernel's field teams are
   // single wavefronts (the rule col_stage applies), two otherwise
-  const int nbuf = (mid && ct * (N1 / E) <= 64) ? 1 : 2;
+  const int nbuf = (mid && team_in_wave(ct * (N1 / E))) ? 1 : 2;
   const size_t lds = ((size_t)N1 + nbuf * (size_t)NF * ct * N1) * sizeof(V) + 64 * sizeof(double);
   const bool direct = mid && a.direct;
   static std::once_flag big_lds[3][kMaxDevices];
@@ -555,7 +555,7 @@ static int launch_row_e(int rt, int count, const OpoInst *insts, const RowArgs &
                         hipEvent_t e1) {
   using V = typename vec2<T>::type;
   const int threads = rt * (N2 / E);
-  const size_t lds = (N2 / E <= 64 ? 1 : 2) * (size_t)rt * (N2 + N2 / 8) * sizeof(V);  // see row_stage
+  const size_t lds = (team_in_wave(N2 / E) ? 1 : 2) * (size_t)rt * (N2 + N2 / 8) * sizeof(V);  // see row_stage
   dim3 grid((a.N1 / rt) * a.nf * (a.xcd_pack ? 8 : 1), count);
   static std::once_flag big_lds[2][kMaxDevices];
   auto kern = a.direct ? &row_stage<T, N2, NF, true, E> : &row_stage<T, N2, NF, false, E>;

@@ -65,3 +65,23 @@ def test_gpu_matches_fixture(gpu, path):
     saved = s.download_saved()["s"]
     slots = saved[: cfg.NRT] if cfg.regime == O.NS else saved[-cfg.NRT:]
     assert rel_l2(slots, z["save_s"]) < 1e-9
+
+
+def test_long_c2_record_is_well_formed():
+    """The 1000-round-trip record of the full-size three-equation workload
+    (tests/golden/make_c2_long.py; the GPU side is tests/test_gpu_long.py):
+    one finite power per round trip, built up by many orders of magnitude,
+    and one projection per row for every field."""
+    from tests.golden import make_c2_long as G
+    ref = json.load(open(G.OUT))
+    pw = np.array(ref["power"])
+    assert pw.shape == (G.NRT,) and np.isfinite(pw).all() and (pw > 0).all()
+    assert pw[-1] > 1e6 * pw[0]
+    for k in "psi":
+        assert len(ref[f"proj_{k}_re"]) == len(ref[f"proj_{k}_im"]) == G.NPROJ
+        assert ref[f"norm_{k}"] > 0
+    # the projector is fixed by its seed: rows of +-1/sqrt(N), nearly orthonormal
+    R = G.projector(1 << 14)
+    assert np.allclose(np.abs(R), 2.0 ** -7)
+    gram = R @ R.T
+    assert np.allclose(np.diag(gram), 1.0) and np.max(np.abs(gram - np.eye(G.NPROJ))) < 0.05
