@@ -127,6 +127,11 @@ struct Engine {
   hipEvent_t fork_ev = nullptr;
 
   ~Engine() {
+    // release on the engine's device (the caller's current device may differ),
+    // then give the caller its device back
+    int prev = -1;
+    const bool switch_dev = stream && hipGetDevice(&prev) == hipSuccess && prev != device;
+    if (switch_dev) (void)hipSetDevice(device);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
     for (auto &g : groups) {
@@ -138,6 +143,7 @@ struct Engine {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
+    if (switch_dev) (void)hipSetDevice(prev);
   }
 };
 
@@ -787,6 +793,7 @@ int opo_upload(opo_solver *s, const double *w, const double *T, const double *Ap
 
 int opo_set_fields(opo_solver *s, const double *Ap, const double *As, const double *Ai) {
   if (!s) return fail(OPO_ERR_INVALID, "null solver");
+  HIPCHK(hipSetDevice(s->E.device));
   Engine &E = s->E;
   Group &g = E.groups[0];
   if (!g.inst[0].uploaded) return fail(OPO_ERR_STATE, "set_fields before upload");
@@ -868,6 +875,7 @@ int opo_use_graph(opo_solver *s, int enable) {
 int opo_crystal_pass(opo_solver *s) {
   using namespace opo;
   if (!s) return fail(OPO_ERR_INVALID, "null solver");
+  HIPCHK(hipSetDevice(s->E.device));
   Engine &E = s->E;
   Group &g = E.groups[0];
   if (!g.inst[0].uploaded) return fail(OPO_ERR_STATE, "crystal_pass before upload");
@@ -886,6 +894,7 @@ int opo_crystal_pass(opo_solver *s) {
 
 int opo_linear_step(opo_solver *s) {
   if (!s) return fail(OPO_ERR_INVALID, "null solver");
+  HIPCHK(hipSetDevice(s->E.device));
   Engine &E = s->E;
   Group &g = E.groups[0];
   if (!g.inst[0].uploaded) return fail(OPO_ERR_STATE, "linear_step before upload");
@@ -901,6 +910,7 @@ int opo_linear_step(opo_solver *s) {
 
 int opo_nl_half(opo_solver *s, int slice, int n_times) {
   if (!s) return fail(OPO_ERR_INVALID, "null solver");
+  HIPCHK(hipSetDevice(s->E.device));
   Engine &E = s->E;
   Group &g = E.groups[0];
   if (!g.inst[0].uploaded) return fail(OPO_ERR_STATE, "nl_half before upload");
@@ -916,6 +926,7 @@ int opo_nl_half(opo_solver *s, int slice, int n_times) {
 int opo_boundary(opo_solver *s) {
   using namespace opo;
   if (!s) return fail(OPO_ERR_INVALID, "null solver");
+  HIPCHK(hipSetDevice(s->E.device));
   Engine &E = s->E;
   Group &g = E.groups[0];
   if (!g.inst[0].uploaded) return fail(OPO_ERR_STATE, "boundary before upload");
@@ -942,6 +953,7 @@ int opo_run(opo_solver *s, int rt0, int n_rt) {
 
 int opo_download(opo_solver *s, double *Ap, double *As, double *Ai) {
   if (!s) return fail(OPO_ERR_INVALID, "null solver");
+  HIPCHK(hipSetDevice(s->E.device));
   Engine &E = s->E;
   Group &g = E.groups[0];
   std::vector<cd> st((size_t)g.NF * g.N);
@@ -960,6 +972,7 @@ int opo_download_saved(opo_solver *s, double *sp, double *ss, double *si) {
 
 int opo_power_history(opo_solver *s, double *out, int n) {
   if (!s || !out) return fail(OPO_ERR_INVALID, "null");
+  HIPCHK(hipSetDevice(s->E.device));
   Engine &E = s->E;
   Group &g = E.groups[0];
   if (n > E.last_nrt) n = E.last_nrt;
@@ -1169,6 +1182,7 @@ int opo_batch_run(opo_batch *b, int max_rt, int check_every, double tol, int *rt
 
 int opo_batch_download(opo_batch *b, int idx, double *Ap, double *As, double *Ai) {
   if (!b || idx < 0 || idx >= b->E.count) return fail(OPO_ERR_INVALID, "bad batch index");
+  HIPCHK(hipSetDevice(b->E.device));
   Engine &E = b->E;
   auto [gi, k] = E.where[idx];
   Group &g = E.groups[gi];
@@ -1185,6 +1199,7 @@ int opo_batch_download(opo_batch *b, int idx, double *Ap, double *As, double *Ai
 
 int opo_batch_power_history(opo_batch *b, int idx, double *out, int n) {
   if (!b || !out || idx < 0 || idx >= b->E.count || n < 0) return fail(OPO_ERR_INVALID, "bad arguments");
+  HIPCHK(hipSetDevice(b->E.device));
   Engine &E = b->E;
   auto [gi, k] = E.where[idx];
   Group &g = E.groups[gi];
@@ -1195,6 +1210,7 @@ int opo_batch_power_history(opo_batch *b, int idx, double *out, int n) {
 
 int opo_batch_power(opo_batch *b, double *out) {
   if (!b || !out) return fail(OPO_ERR_INVALID, "null");
+  HIPCHK(hipSetDevice(b->E.device));
   Engine &E = b->E;
   for (auto &g : E.groups) {
     const int n = (int)g.members.size();
