@@ -34,6 +34,16 @@ def workload_c2(N=1 << 14, NZ=100, NRT=5000):
                   pm_mode=P.PM_THREE_WAVE)
 
 
+def workload_c1(N=1 << 12, NZ=32, NRT=2000):
+    """Continuous-wave two-equation MgO:PPLN OPO (532 -> 1064 nm), 5 mm
+    crystal, 2 W of pump (the threshold ratio follows from the threshold)."""
+    import cuopo_amd as P
+    a = P.Args(three_eqs=False, regime=P.CW, crystal=P.PPLN, N=N, NZ=NZ, NRT=NRT, lp_nm=532.0, Lambda=6.97,
+               Nth=1.0, R_pct=98.0, lcr_mm=5.0)
+    a.Nth = 2.0 / P.derive(a).Pth
+    return a
+
+
 def workload_c5(N=1 << 18, NZ=200, NRT=40):
     """Nanosecond-pulsed three-equation MgO:PPLN OPO, EOM and GDD compensation
     on (4x threshold, EOM depth 0.8 rad, modulation at 4 x FSR), 5 mm crystal."""
@@ -44,6 +54,9 @@ def workload_c5(N=1 << 18, NZ=200, NRT=40):
 
 
 TUNING = None  # (col_threads, row_threads, xcd_pack, phase_mode[, col_spt, row_spt]) instead of autotuning
+
+
+SWEEP_RTS = []  # (points, round trips each needed) of the batched cold sweep, for the CPU estimate
 
 
 def log(msg):
@@ -135,28 +148,115 @@ def load_traffic(name):
         return None
 
 
-def cpu_baseline(args_c2, seconds=12.0):
-    """FP64 oracle (a scalar C transcription of the reference kernels with the
-    project's radix-2 FFT), one core, on round trips of the same workload."""
+def cpu_host():
+    """(CPU model, cores this process may use, threads the all-cores run uses:
+    the affinity set capped at OMP_NUM_THREADS, 16 when unset)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    ncpu = len(os.sched_getaffinity(0))
+    return model, ncpu, max(1, min(ncpu, int(os.environ.get("OMP_NUM_THREADS") or 16)))
+
+
+def oracle_instance(args, seed=1):
+    """The FP64 oracle's inputs for one workload (seeded 1e-20 noise)."""
     from oracle import oracle as O
-    cfg = O.Config(**{k: getattr(args_c2, k) for k in args_c2.__dataclass_fields__})
+    cfg = O.Config(**{k: getattr(args, k) for k in args.__dataclass_fields__})
     p = O.setup(cfg)
     T, w = O.grids(p)
     Ap_in = O.pump_input(p)
-    As = O.noise(1, 0, 1e-20, p.N)
-    Ai = O.noise(1, 1, 1e-20, p.N)
-    Ap = Ap_in.copy()
+    As = O.noise(seed, 0, 1e-20, p.N)
+    Ai = O.noise(seed, 1, 1e-20, p.N) if cfg.three_eqs else None
+    return p, w, T, Ap_in, Ap_in[: p.N].copy(), As, Ai
+
+
+def oracle_rate(inst, seconds, start=None):
+    """Round trips/s of the oracle on one thread over at least `seconds`
+    (timer around the round-trip calls only)."""
+    from oracle import oracle as O
+    p, w, T, Ap_in, Ap, As, Ai = inst
+    if start is not None:
+        start.wait()
     n = 0
     t0 = time.perf_counter()
     while True:
-        O.run(p, w, T, Ap_in, Ap, As, Ai, n, 1)
+        O.run(p, w, T, Ap_in, Ap, As, Ai, n % p.NRT, 1)
         n += 1
-        if time.perf_counter() - t0 >= seconds:
-            break
-    dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "round trips/s", "cores": 1, "kind": "port",
-            "sample": f"{n} round trips of the same workload ({dt:.1f} s), FP64 oracle "
-                      "(radix-2 FFT, no FFTW on the host), single thread"}
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            return n / dt
+
+
+def median_rate(inst, seconds, reps=5):
+    import numpy as np
+    return float(np.median([oracle_rate(inst, seconds / reps) for _ in range(reps)]))
+
+
+def cpu_baseline(args_c2, seconds=12.0):
+    """Headline CPU baseline: the FP64 oracle (a scalar C transcription of the
+    reference kernels with the project's radix-2 FFT) on one core, round
+    trips of the same workload, median of 5 timings."""
+    model, ncpu, _ = cpu_host()
+    v = median_rate(oracle_instance(args_c2), seconds)
+    return {"value": v, "unit": "round trips/s", "cores": 1, "kind": "port",
+            "sample": f"median of 5 timings of >= {seconds / 5:.1f} s each, round trips of the same workload, "
+                      f"FP64 oracle (radix-2 FFT, no FFTW on the host), single thread on {model} "
+                      f"({ncpu} cores available)"}
+
+
+def cpu_extra(seconds=8.0, sweep=None):
+    """The other CPU baselines: C1 and C5 single instance on one core (C5 timed
+    on 20 slices per pass and scaled to 200: the boundary work is counted ten
+    times, so the C5 figure is slightly low), and the sweep family with one
+    instance per core on all cores (threads over the oracle, which releases
+    the interpreter lock in its C calls): round trips/s per core per grid size,
+    and, when the sweep section ran, the sweep points/s those rates give for
+    the round trips each point needed in the batched run, with perfect load
+    balance over the cores."""
+    import threading
+    import numpy as np
+    from cuopo_amd import sweep as SW
+    from cuopo_amd.surrogate import sweep_args
+    model, ncpu, nthr = cpu_host()
+    out = {"host": model, "cores_available": ncpu, "threads_all_cores": nthr}
+    out["c1_1core_round_trips_per_s"] = median_rate(oracle_instance(workload_c1()), seconds / 2)
+    c5 = workload_c5(NRT=40)
+    c5.NZ = 20
+    out["c5_1core_round_trips_per_s"] = median_rate(oracle_instance(c5), seconds / 2, reps=3) * 20 / 200
+    pts = SW.sweep_points(256, seed=3)
+    firsts = {N: next(p for p in pts if p[0] == N) for N in SW.SWEEP_N}
+    Ns = [SW.SWEEP_N[k % len(SW.SWEEP_N)] for k in range(nthr)]
+    insts = []
+    for N in Ns:
+        _, nth, delta, lcr, gdd, R = firsts[N]
+        insts.append(oracle_instance(sweep_args(nth, delta, lcr, gdd, R, N=N, NZ=100, NRT=5000)))
+    rates = [0.0] * nthr
+    start = threading.Barrier(nthr)
+
+    def work(k):
+        rates[k] = oracle_rate(insts[k], seconds, start)
+
+    th = [threading.Thread(target=work, args=(k,)) for k in range(nthr)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    per_n = {N: float(np.mean([r for r, n in zip(rates, Ns) if n == N])) for N in SW.SWEEP_N if N in Ns}
+    c3 = {"round_trips_per_s_per_core": {str(N): v for N, v in per_n.items()}, "threads": nthr}
+    if sweep is not None and len(per_n) == len(SW.SWEEP_N):
+        sweep_pts, rts = sweep
+        core_s = float(sum(rts[k] / per_n[p[0]] for k, p in enumerate(sweep_pts)))
+        c3["est_sweep_points_per_s"] = len(sweep_pts) / (core_s / nthr)
+        c3["estimate"] = ("sum over the sweep points of the round trips each ran in the batched FP32C cold run / "
+                          "the per-core oracle rate at its grid size, spread evenly over the threads")
+    out["c3_sweep_all_cores"] = c3
+    return out
 
 
 def sweep_section(weights):
@@ -182,6 +282,7 @@ def sweep_section(weights):
 
     cold = SW.run_batched(pts, precision=P.FP32C)
     out["batched_fp32c_cold"], cold_rts = summarise(cold)
+    SWEEP_RTS[:] = [(pts, cold_rts)]
     cold_power, cold_states, ap0 = cold["power"], cold["states"], cold["ap0"]
     cold["batch"].close()
     # plain FP64 per-instance solver: measured round-trip time per grid size
@@ -328,6 +429,12 @@ def main():
     if not a.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(args, a.cpu_seconds)
+        sweep_b = out["extra"].get("sweep", {}).get("batched_fp32c_cold")
+        log("cpu baselines: C1, C5, sweep on all cores")
+        out["extra"]["cpu"] = cpu_extra(sweep=SWEEP_RTS[0] if SWEEP_RTS else None)
+        c3 = out["extra"]["cpu"]["c3_sweep_all_cores"]
+        if sweep_b and "est_sweep_points_per_s" in c3:
+            c3["gpu_batched_fp32c_over_all_cores"] = sweep_b["points_per_s"] / c3["est_sweep_points_per_s"]
     print(json.dumps(out))
 
 
