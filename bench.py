@@ -14,6 +14,7 @@ stream) and a CPU baseline (the FP64 oracle on this host's cores).
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -284,6 +285,7 @@ def sweep_section(weights):
     out["batched_fp32c_cold"], cold_rts = summarise(cold)
     SWEEP_RTS[:] = [(pts, cold_rts)]
     cold_power, cold_states, ap0 = cold["power"], cold["states"], cold["ap0"]
+    arms = {"batched_fp32c_cold": (cold_states, cold_power)}
     cold["batch"].close()
     # plain FP64 per-instance solver: measured round-trip time per grid size
     per_rt = {}
@@ -321,7 +323,54 @@ def sweep_section(weights):
                  "max": float(rel[both].max()) if both.any() else None}
             out[key]["same_finish_state_as_cold"] = int((w["states"] == cold_states).sum())
             out[key]["speedup_vs_per_instance"] = out[key]["points_per_s"] / (len(pts) / est)
+            arms[key] = (w["states"], w["power"])
             w["batch"].close()
+    out["fp64_cold_check"] = fp64_steady_check(pts, arms)
+    return out
+
+
+def fp64_steady_check(pts, arms, max_rt=10000, seed=1000, sigma=1e-20, every=50, tol=1e-6):
+    """The sweep target's accuracy check.  The points that finished steady in
+    any FP32C arm run again in FP64 from the cold run's noise (the same seed
+    per point) for max_rt round trips.  From each FP64 power history: the
+    power where the convergence test (|dP| <= tol P over `every` round trips)
+    first passes, P_test, and the power after max_rt round trips, P_end.
+    Reported: how far FP64 itself moves after its test passes, and each FP32C
+    arm's steady power against P_test and against P_end."""
+    import numpy as np
+    import cuopo_amd as P
+    from cuopo_amd import sweep as SW
+    idx = sorted({int(k) for st, _ in arms.values() for k in np.flatnonzero(st == 1)})
+    if not idx:
+        return {"points": 0}
+    sub = [pts[k] for k in idx]
+    warm = [(P.noise(seed + k, 0, sigma, pts[k][0]), P.noise(seed + k, 1, sigma, pts[k][0])) for k in idx]
+    b, _ = SW.make_batch(sub, max_rt=max_rt, precision=P.FP64, warm=warm, autotune=True)
+    b.run_ex(max_rt, check_every=0, tol=0.0)
+    hist = np.array([b.power_history(j, max_rt) for j in range(len(sub))])
+    b.close()
+    p_test = np.full(len(idx), np.nan)
+    for j, h in enumerate(hist):
+        c = h[every - 1::every]
+        ok = np.flatnonzero(np.abs(c[1:] - c[:-1]) <= tol * c[1:])
+        if ok.size:
+            p_test[j] = c[ok[0] + 1]
+    p_end = hist[:, -1]
+    has = np.isfinite(p_test)
+    drift = np.abs(p_test[has] - p_end[has]) / p_end[has]
+
+    def stats(rel):
+        return {"median": float(np.median(rel)) if rel.size else None, "max": float(rel.max()) if rel.size else None,
+                "within_1e-4": int((rel <= 1e-4).sum())}
+
+    out = {"points": len(idx), "max_rt": max_rt, "fp64_test_passed": int(has.sum()),
+           "fp64_drift_after_test": stats(drift), "arms": {}}
+    for key, (st, pw) in arms.items():
+        js = [j for j, k in enumerate(idx) if st[k] == 1 and has[j]]
+        a = np.array([pw[idx[j]] for j in js])
+        out["arms"][key] = {"points": len(js),
+                            "vs_fp64_at_test": stats(np.abs(a - p_test[js]) / p_test[js]) if js else None,
+                            "vs_fp64_end": stats(np.abs(a - p_end[js]) / p_end[js]) if js else None}
     return out
 
 
@@ -418,6 +467,14 @@ def main():
                   "graph_slice_us": 1e3 * dev_ms / (a.steps * args.NZ),
                   "graph_slice_frac_of_hbm": 2 * Bz / (1e-3 * dev_ms / (a.steps * args.NZ)) / HBM_PEAK,
                   "copy_probe_GBps": copy_gbps, "tuning": tuning,
+                  # whole round trip: B_rt x round trips/s against the HBM peak and the
+                  # measured copy rate (a two-pass slice moves every field twice, so
+                  # this fraction is at most half the per-launch one)
+                  "round_trip_frac_of_hbm": b_rt(N, args.NZ, nf, 2, sc) * rts / HBM_PEAK,
+                  "round_trip_frac_of_copy": b_rt(N, args.NZ, nf, 2, sc) * rts / (copy_gbps * 1e9),
+                  # secondary: nf (10 log2 N + 156) flops per sample and slice (FFT pair,
+                  # NL step, linear phase) at the measured rate, against the FP64 peak
+                  "algorithmic_fp64_tflops": nf * (10 * math.log2(N) + 156) * N * args.NZ * rts / 1e12,
                   "intracavity_power_end": float(np.sum(np.abs(As) ** 2))},
     }
     if not a.no_shapes:
