@@ -10,7 +10,12 @@ Gaussian noise.
 
 Prints one JSON line: round trips/s, the roofline of the dominant kernel
 (per-launch kernel time measured with start/stop events on the solver's
-stream) and a CPU baseline (the FP64 oracle on this host's cores).
+stream) and a CPU baseline (the FP64 oracle on one of this host's cores).
+Under "extra": the per-slice time inside the round-trip graph, the round-trip
+HBM fraction and FP64 flop rate, the N = 2^18 single instance and the 256
+instance FP64 batch, the 256-point sweep (batched FP32C against the plain
+per-instance FP64 solver, with the steady-power check against FP64 cold
+starts), and the CPU baselines of the other workloads (one core, all cores).
 """
 import argparse
 import json
