@@ -67,17 +67,19 @@ def test_gpu_matches_fixture(gpu, path):
     assert rel_l2(slots, z["save_s"]) < 1e-9
 
 
-def test_long_c2_record_is_well_formed():
-    """The 1000-round-trip record of the full-size three-equation workload
-    (tests/golden/make_c2_long.py; the GPU side is tests/test_gpu_long.py):
-    one finite power per round trip, built up by many orders of magnitude,
-    and one projection per row for every field."""
-    from tests.golden import make_c2_long as G
-    ref = json.load(open(G.OUT))
+@pytest.mark.parametrize("name", ["c2", "c1"])
+def test_long_record_is_well_formed(name):
+    """The long-run records of full-size workloads (tests/golden/make_long.py;
+    the GPU side is tests/test_gpu_long.py): one finite power per round trip,
+    built up by many orders of magnitude, and one projection per row for
+    every field."""
+    from tests.golden import make_long as G
+    ref = json.load(open(G.path(name)))
+    nrt = G.WORKLOADS[name]["nrt"]
     pw = np.array(ref["power"])
-    assert pw.shape == (G.NRT,) and np.isfinite(pw).all() and (pw > 0).all()
+    assert pw.shape == (nrt,) and np.isfinite(pw).all() and (pw > 0).all()
     assert pw[-1] > 1e6 * pw[0]
-    for k in "psi":
+    for k in ("psi" if name == "c2" else "ps"):
         assert len(ref[f"proj_{k}_re"]) == len(ref[f"proj_{k}_im"]) == G.NPROJ
         assert ref[f"norm_{k}"] > 0
     # the projector is fixed by its seed: rows of +-1/sqrt(N), nearly orthonormal
