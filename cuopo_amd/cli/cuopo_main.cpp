@@ -14,6 +14,8 @@
 //   --out=<dir>                       output directory (default: cwd)
 //   --fp32c                           FP32-complex variant
 //   --no-inputs                       skip the *_input_*.dat files
+//   --format=dat|bin|both             text .dat files (default), raw float64
+//                                     .bin side-cars, or both
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -69,7 +71,7 @@ void usage(const char *prog) {
   std::fprintf(stderr,
                "usage: %s <LP_nm> <T_C> <Lambda_um> <N> <R_%%> <delta_pi> <GDD_%%> <TOD_%%> <UPM> <beta_pi> <df> "
                "<Lcr_mm> [--N=] [--nz=] [--nrt=] [--seed=] [--noise=] [--ls=] [--pm=3wave] [--out=] [--fp32c] "
-               "[--no-inputs]\n",
+               "[--no-inputs] [--format=dat|bin|both]\n",
                prog);
 }
 
@@ -127,7 +129,9 @@ int main(int argc, char *argv[]) {
     else if (f == "--pm=3wave") a.pm_mode = OPO_PM_THREE_WAVE;
     else if (f == "--fp32c") precision = OPO_FP32C;
     else if (f == "--no-inputs") save_inputs = false;
-    else { usage(argv[0]); return 1; }
+    else if (const char *v = val("--format=")) {
+      if (!opo_cli::set_format(v)) { usage(argv[0]); return 1; }
+    } else { usage(argv[0]); return 1; }
   }
 
   opo_derived d{};

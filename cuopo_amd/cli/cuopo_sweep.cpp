@@ -9,7 +9,7 @@
 //
 //   cuopo_sweep <spec> [--out=DIR] [--N=] [--nz=] [--nrt=] [--seed=] [--noise=]
 //               [--ls=] [--pm=3wave] [--fp32c] [--no-inputs] [--max-batch=K]
-//               [--mem-gb=G] [--dry-run]
+//               [--mem-gb=G] [--format=dat|bin|both] [--dry-run]
 //
 // Spec file: one `KEY = v1 v2 ...` per line ('#' starts a comment), the
 // arrays of cuOPO.sh:29,62-63,68-88 with their names:
@@ -171,7 +171,7 @@ int main(int argc, char *argv[]) {
   if (argc < 2) {
     std::fprintf(stderr,
                  "usage: %s <spec> [--out=DIR] [--N=] [--nz=] [--nrt=] [--seed=] [--noise=] [--ls=] [--pm=3wave] "
-                 "[--fp32c] [--no-inputs] [--max-batch=K] [--mem-gb=G] [--dry-run]\n",
+                 "[--fp32c] [--no-inputs] [--max-batch=K] [--mem-gb=G] [--format=dat|bin|both] [--dry-run]\n",
                  argv[0]);
     return 1;
   }
@@ -195,7 +195,9 @@ int main(int argc, char *argv[]) {
     else if (f == "--fp32c") fl.precision = OPO_FP32C;
     else if (f == "--no-inputs") fl.inputs = false;
     else if (f == "--dry-run") fl.dry = true;
-    else die("unknown flag " + f);
+    else if (const char *v = val("--format=")) {
+      if (!opo_cli::set_format(v)) die("--format is dat, bin or both");
+    } else die("unknown flag " + f);
   }
   if (!fl.have_seed) fl.seed = (uint64_t)std::chrono::system_clock::now().time_since_epoch().count();
 

@@ -11,7 +11,41 @@
 
 namespace opo_cli {
 
+namespace {
+Format g_format = Format::Dat;
+
+// raw little-endian float64, the .dat file's values in its order
+bool write_bin(const std::string &path, const double *v, size_t n) {
+  FILE *f = std::fopen(path.c_str(), "wb");
+  if (!f) return false;
+  const bool ok = std::fwrite(v, sizeof(double), n, f) == n;
+  return std::fclose(f) == 0 && ok;
+}
+
+bool write_dat(const std::string &path, const double *v, size_t n, int threads);
+}  // namespace
+
+bool set_format(const std::string &name) {
+  if (name == "dat") g_format = Format::Dat;
+  else if (name == "bin") g_format = Format::Bin;
+  else if (name == "both") g_format = Format::Both;
+  else return false;
+  return true;
+}
+
 bool write_real(const std::string &path, const double *v, size_t n, int threads) {
+  bool ok = true;
+  if (g_format != Format::Bin) ok = write_dat(path, v, n, threads);
+  if (g_format != Format::Dat) {
+    const std::string stem = path.size() > 4 && path.compare(path.size() - 4, 4, ".dat") == 0
+                                 ? path.substr(0, path.size() - 4) : path;
+    ok = write_bin(stem + ".bin", v, n) && ok;
+  }
+  return ok;
+}
+
+namespace {
+bool write_dat(const std::string &path, const double *v, size_t n, int threads) {
   FILE *f = std::fopen(path.c_str(), "w");
   if (!f) return false;
   // format blocks of values in parallel, write the blocks in order
@@ -41,6 +75,7 @@ bool write_real(const std::string &path, const double *v, size_t n, int threads)
   }
   return std::fclose(f) == 0 && ok;
 }
+}  // namespace
 
 bool write_complex(const std::string &stem, const double *v, size_t n, int threads) {
   std::vector<double> re(n), im(n);

@@ -11,9 +11,17 @@
 
 namespace opo_cli {
 
+// Output files: the reference's text .dat files, a binary side-car per file
+// (X.bin next to X.dat: raw little-endian float64, the same values in the same
+// order, e.g. numpy.fromfile(path, "<f8")), or both.  Process-wide; set once
+// from the command line (--format=dat|bin|both; false for any other name).
+enum class Format { Dat, Bin, Both };
+bool set_format(const std::string &name);
+
 // One value per line at 20 significant digits (SaveVectorReal,
 // files.cuh:21-40).  Formatting runs on `threads` host threads (0 = all
-// hardware threads), the file is written in order.
+// hardware threads), the file is written in order.  `path` names the .dat
+// file; the side-car, when selected, replaces its ".dat" by ".bin".
 bool write_real(const std::string &path, const double *v, size_t n, int threads = 0);
 // Interleaved complex -> stem_r.dat / stem_i.dat (SaveVectorComplex,
 // files.cuh:43-62).

@@ -20,6 +20,11 @@ import numpy as np
 
 
 def read_dat(path):
+    """A .dat file of the run, or its binary side-car (the same name with
+    .bin: raw little-endian float64, written with --format=bin|both) when the
+    .dat file is absent."""
+    if not os.path.exists(path) and path.endswith(".dat") and os.path.exists(path[:-4] + ".bin"):
+        return np.fromfile(path[:-4] + ".bin", dtype="<f8")
     return np.loadtxt(path, dtype=np.float64)
 
 

@@ -30,6 +30,13 @@ def test_usage_on_missing_arguments(tmp_path):
     assert "usage:" in r.stderr
 
 
+def test_bad_output_format_is_a_usage_error(tmp_path):
+    r = run([exe("cuOPO_cw_ppln_2eq"), "532", "0", "6.97", "4", "98", "0", "0", "0", "0", "0", "0", "5",
+             "--format=csv"], tmp_path)
+    assert r.returncode == 1
+    assert "usage:" in r.stderr and "--format=dat|bin|both" in r.stderr
+
+
 def test_parameter_dump_and_error_convention(tmp_path):
     """The parameter dump carries the setup scalars; a rejected grid exits 1
     with the reference's 'Error: file:line, code:, reason:' line."""
@@ -82,3 +89,28 @@ def test_cli_outputs_match_oracle(gpu, tmp_path, variant):
     assert Tp.size == p.SIZEL and np.allclose(Tp, Tpo, rtol=1e-15)
     assert np.allclose(np.loadtxt(tmp_path / "T.dat"), T, rtol=1e-15)
     assert np.allclose(np.loadtxt(tmp_path / "freq.dat"), Fpo, rtol=1e-15)
+
+
+@pytest.mark.gpu
+def test_binary_side_cars_equal_the_text_files(gpu, tmp_path):
+    """--format=both writes X.bin (raw float64) beside every X.dat with the
+    same values; --format=bin writes only the .bin files, identical bytes for
+    the same seeded run."""
+    argv = ["532", "0", "6.97", "4", "98", "0.05", "0", "0", "0", "0.25", "4", "5",
+            "--N=256", "--nz=2", "--nrt=70", "--seed=3", "--noise=1e-6"]
+    both, only = tmp_path / "both", tmp_path / "bin"
+    both.mkdir(); only.mkdir()
+    r = run([exe("cuOPO_cw_ppln_3eq")] + argv + ["--format=both"], both)
+    assert r.returncode == 0, r.stderr[-2000:]
+    dats = sorted(f for f in os.listdir(both) if f.endswith(".dat"))
+    assert len(dats) == 15  # three inputs, three outputs (re/im each), Tp, freq, T
+    for f in dats:
+        txt = np.loadtxt(both / f)
+        binv = np.fromfile(both / (f[:-4] + ".bin"), dtype="<f8")
+        assert binv.shape == txt.shape and np.array_equal(binv, txt), f
+    r = run([exe("cuOPO_cw_ppln_3eq")] + argv + ["--format=bin"], only)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert not [f for f in os.listdir(only) if f.endswith(".dat")]
+    for f in dats:
+        b = f[:-4] + ".bin"
+        assert (only / b).read_bytes() == (both / b).read_bytes(), b

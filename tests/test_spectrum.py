@@ -55,3 +55,17 @@ def test_mismatched_files_are_rejected(tmp_path):
     write_run(tmp_path, A, np.arange(4 * N), np.arange(4 * N), N)
     with pytest.raises(SystemExit):
         SP.main([str(tmp_path), "--slot", "4"])
+
+
+def test_binary_side_cars_are_read_when_the_text_files_are_absent(tmp_path):
+    N, nslots = 64, 4
+    SIZEL = N * nslots
+    A = np.exp(2j * np.pi * 5 * np.arange(SIZEL) / SIZEL)
+    Fp = np.linspace(-0.5, 0.5, SIZEL)
+    for name, v in (("signal_output_r", A.real), ("signal_output_i", A.imag), ("freq", Fp),
+                    ("T", np.linspace(-0.5, 0.5, N))):
+        v.astype("<f8").tofile(tmp_path / f"{name}.bin")
+    out = SP.main([str(tmp_path)])
+    S = np.loadtxt(out)
+    assert int(np.argmax(S)) == SIZEL // 2 + 5
+    assert S.sum() == pytest.approx(1.0, rel=1e-12)
