@@ -16,6 +16,12 @@
 //   --no-inputs                       skip the *_input_*.dat files
 //   --format=dat|bin|both             text .dat files (default), raw float64
 //                                     .bin side-cars, or both
+//   --gdd-fs2=<fs^2>                  intracavity GDD in fs^2 instead of argv7's
+//                                     compensation percentage
+//   --converge[=<tol>]                CW: stop the round trips once the signal
+//                                     power changes by < tol (1e-6) relative
+//                                     over 50 of them, then run the 64 saved
+//                                     round trips
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -71,7 +77,7 @@ void usage(const char *prog) {
   std::fprintf(stderr,
                "usage: %s <LP_nm> <T_C> <Lambda_um> <N> <R_%%> <delta_pi> <GDD_%%> <TOD_%%> <UPM> <beta_pi> <df> "
                "<Lcr_mm> [--N=] [--nz=] [--nrt=] [--seed=] [--noise=] [--ls=] [--pm=3wave] [--out=] [--fp32c] "
-               "[--no-inputs] [--format=dat|bin|both]\n",
+               "[--no-inputs] [--format=dat|bin|both] [--gdd-fs2=] [--converge[=tol]]\n",
                prog);
 }
 
@@ -113,6 +119,8 @@ int main(int argc, char *argv[]) {
   std::string out = ".";
   int precision = OPO_FP64;
   bool save_inputs = true;
+  double gdd_fs2 = 0.0, converge = 0.0;
+  bool have_gdd_fs2 = false;
   for (int k = 13; k < argc; ++k) {
     const std::string f = argv[k];
     auto val = [&](const char *key) -> const char * {
@@ -131,7 +139,16 @@ int main(int argc, char *argv[]) {
     else if (f == "--no-inputs") save_inputs = false;
     else if (const char *v = val("--format=")) {
       if (!opo_cli::set_format(v)) { usage(argv[0]); return 1; }
-    } else { usage(argv[0]); return 1; }
+    } else if (const char *v = val("--gdd-fs2=")) { gdd_fs2 = std::atof(v); have_gdd_fs2 = true; }
+    else if (const char *v = val("--converge=")) converge = std::atof(v);
+    else if (f == "--converge") converge = 1e-6;
+    else { usage(argv[0]); return 1; }
+  }
+  if (have_gdd_fs2) {
+    // GDD = -epsilon * beta2_s * Lcr (src/cuOPO.cu:142): the percentage that gives it
+    opo_derived d0{};
+    CHECK(opo_derive(&a, &d0));
+    a.gdd_pct = -100.0 * (gdd_fs2 * 1e-6) / (d0.b2s * d0.Lcr);
   }
 
   opo_derived d{};
@@ -153,7 +170,8 @@ int main(int argc, char *argv[]) {
   }
 
   // parameter dump, src/cuOPO.cu:289-369
-  opo_cli::print_params(std::cout, d, a, std::atof(argv[7]), std::atof(argv[8]), Fp[d.SIZEL - 1]);
+  opo_cli::print_params(std::cout, d, a, have_gdd_fs2 ? a.gdd_pct : std::atof(argv[7]), std::atof(argv[8]),
+                        Fp[d.SIZEL - 1]);
 
   // solver (device allocation + uploads, src/cuOPO.cu:379-442)
   opo_params p{};
@@ -165,14 +183,37 @@ int main(int argc, char *argv[]) {
   CHECK(opo_enable_save(s, 1));
   std::cout << "Starting main loop on CPU & GPU...\n" << std::endl;
 
-  // round-trip loop (src/cuOPO.cu:474-610) in chunks of 500 for the progress line
-  for (int nn = 0; nn < d.NRT;) {
-    std::cout << "#round trip: " << nn << " - Completed " << (long long)nn * 100 / d.NRT << "%" << "\t\r" << std::flush;
-    const int chunk = std::min(500, d.NRT - nn);
-    CHECK(opo_run(s, nn, chunk));
-    nn += chunk;
+  if (converge > 0 && d.regime == OPO_CW) {
+    // chunks of 50 round trips until the power settles (or only the saved
+    // tail of NRT is left), then the round trips the reference saves
+    const int tail = std::min(d.NRT, d.Nrts);
+    std::vector<double> hist(50);
+    double prev = -1.0;
+    int nn = 0;
+    bool steady = false;
+    while (nn < d.NRT - tail && !steady) {
+      const int chunk = std::min(50, d.NRT - tail - nn);
+      CHECK(opo_run(s, nn, chunk));
+      CHECK(opo_power_history(s, hist.data(), chunk));
+      nn += chunk;
+      const double P = hist[chunk - 1];
+      steady = chunk == 50 && prev > 0 && std::fabs(P - prev) <= converge * P;
+      prev = P;
+    }
+    std::cout << (steady ? "Steady after " : "Not steady after ") << nn << " round trips" << std::endl;
+    CHECK(opo_run(s, d.NRT - tail, tail));
+  } else {
+    // round-trip loop (src/cuOPO.cu:474-610) in chunks of 500 for the progress line
+    for (int nn = 0; nn < d.NRT;) {
+      std::cout << "#round trip: " << nn << " - Completed " << (long long)nn * 100 / d.NRT << "%" << "\t\r"
+                << std::flush;
+      const int chunk = std::min(500, d.NRT - nn);
+      CHECK(opo_run(s, nn, chunk));
+      nn += chunk;
+    }
+    std::cout << "#round trip: " << d.NRT - 1 << " - Completed " << (long long)(d.NRT - 1) * 100 / d.NRT << "%"
+              << std::endl;
   }
-  std::cout << "#round trip: " << d.NRT - 1 << " - Completed " << (long long)(d.NRT - 1) * 100 / d.NRT << "%" << std::endl;
 
   // outputs, src/cuOPO.cu:620-635
   std::cout << "\nSaving time and frequency vectors...\n" << std::endl;

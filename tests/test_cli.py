@@ -8,7 +8,7 @@ import pytest
 
 from oracle import oracle as O
 from tests.conftest import ROOT
-from tests.helpers import rel_l2
+from tests.helpers import rel_l2, to_args
 
 CLI = os.path.join(ROOT, "cuopo_amd", "cli")
 
@@ -28,6 +28,14 @@ def test_usage_on_missing_arguments(tmp_path):
     r = run([exe("cuOPO_cw_ppln_2eq"), "532", "0", "6.97"], tmp_path)
     assert r.returncode == 1
     assert "usage:" in r.stderr
+
+
+def test_bad_flags_are_usage_errors(tmp_path):
+    for bad in ("--format=csv", "--converge-now", "--gdd=3"):
+        r = run([exe("cuOPO_cw_ppln_2eq"), "532", "0", "6.97", "4", "98", "0", "0", "0", "0", "0", "0", "5", bad],
+                tmp_path)
+        assert r.returncode == 1 and "usage:" in r.stderr, bad
+    assert "--converge[=tol]" in r.stderr and "--gdd-fs2=" in r.stderr
 
 
 def test_bad_output_format_is_a_usage_error(tmp_path):
@@ -114,3 +122,80 @@ def test_binary_side_cars_equal_the_text_files(gpu, tmp_path):
     for f in dats:
         b = f[:-4] + ".bin"
         assert (only / b).read_bytes() == (both / b).read_bytes(), b
+
+
+@pytest.mark.gpu
+def test_absolute_gdd_equals_its_percentage(gpu, tmp_path):
+    """--gdd-fs2 sets GDD = -eps beta2_s Lcr through the percentage that gives
+    it: the outputs equal a run given that percentage as argv7."""
+    base = ["532", "0", "6.97", "4", "98", "0.05"]
+    tail = ["0", "0", "0.25", "4", "5", "--N=256", "--nz=2", "--nrt=70", "--seed=3", "--noise=1e-6",
+            "--format=bin", "--no-inputs"]
+    cfg = O.Config(three_eqs=True, regime=O.CW, crystal=O.PPLN, N=256, NZ=2, NRT=70, lp_nm=532, Lambda=6.97,
+                   Nth=4, R_pct=98, delta=0.05, lcr_mm=5)
+    p = O.setup(cfg)
+    gdd_fs2 = 300.0
+    pct = -100.0 * gdd_fs2 * 1e-6 / (p.b2s * p.Lcr)
+    a, b = tmp_path / "fs2", tmp_path / "pct"
+    a.mkdir(); b.mkdir()
+    r = run([exe("cuOPO_cw_ppln_3eq")] + base + ["0"] + tail + [f"--gdd-fs2={gdd_fs2}"], a)
+    assert r.returncode == 0, r.stderr[-2000:]
+    r = run([exe("cuOPO_cw_ppln_3eq")] + base + [f"{pct:.17g}"] + tail, b)
+    assert r.returncode == 0, r.stderr[-2000:]
+    for f in ("signal_output_r.bin", "signal_output_i.bin", "idler_output_r.bin"):
+        x, y = np.fromfile(a / f, "<f8"), np.fromfile(b / f, "<f8")
+        assert np.array_equal(x, y), f
+    # and the GDD really acts: without it the signal differs
+    c = tmp_path / "none"
+    c.mkdir()
+    r = run([exe("cuOPO_cw_ppln_3eq")] + base + ["0"] + tail, c)
+    assert r.returncode == 0
+    assert not np.array_equal(np.fromfile(c / "signal_output_r.bin", "<f8"),
+                              np.fromfile(a / "signal_output_r.bin", "<f8"))
+
+
+@pytest.mark.gpu
+def test_converge_stops_then_saves_the_tail(gpu, tmp_path):
+    """--converge=1e-3 (CW): round trips in chunks of 50 until the signal
+    power changes by < 1e-3 over 50 of them, then the 64 round trips the
+    reference saves.  The same schedule through the library (the same untuned
+    kernels, deterministic) must stop at the same round trip and give
+    identical saved slots; the library itself is held to the oracle by the
+    parity tests (this multimode regime amplifies any rounding difference, so
+    the oracle's own stopping round trip is not a stable reference here)."""
+    N, NZ, NRT, tol = 256, 4, 2000, 1e-3
+    argv = ["532", "0", "6.97", "2", "90", "0", "0", "0", "0", "0", "0", "5", f"--N={N}", f"--nz={NZ}",
+            f"--nrt={NRT}", "--seed=3", "--noise=1e-3", f"--converge={tol}", "--format=bin"]
+    r = run([exe("cuOPO_cw_ppln_3eq")] + argv, tmp_path)
+    assert r.returncode == 0, r.stderr[-2000:]
+    line = [l for l in r.stdout.splitlines() if "teady after" in l]
+    assert len(line) == 1 and line[0].startswith("Steady after "), line
+    K = int(line[0].split()[2])
+    cfg = O.Config(three_eqs=True, regime=O.CW, crystal=O.PPLN, N=N, NZ=NZ, NRT=NRT, lp_nm=532, Lambda=6.97,
+                   Nth=2, R_pct=90, delta=0.0, lcr_mm=5)
+    p = O.setup(cfg)
+    T, w = O.grids(p)
+    # the executable's own inputs (its noise generator and the oracle's agree to ~1e-16, and this
+    # regime amplifies that)
+    rb = lambda stem: (np.fromfile(tmp_path / f"{stem}_r.bin", "<f8")
+                       + 1j * np.fromfile(tmp_path / f"{stem}_i.bin", "<f8"))
+    Ap_in, As, Ai = rb("pump_input"), rb("signal_input"), rb("idler_input")
+    P = gpu
+    s = P.Solver(P.params_from(P.derive(to_args(cfg)), 0))
+    s.upload(w, T, Ap_in, Ap_in[:N].copy(), As, Ai)
+    s.enable_save(True)
+    nn, prev = 0, -1.0
+    while nn < NRT - 64:
+        s.run(nn, 50)
+        P_end = s.power_history(50)[-1]
+        nn += 50
+        if prev > 0 and abs(P_end - prev) <= tol * P_end:
+            break
+        prev = P_end
+    assert 100 <= K < NRT - 64 and nn == K
+    s.run(NRT - 64, 64)
+    sv = s.download_saved()
+    s.close()
+    for stem, ref in (("signal_output", sv["s"]), ("idler_output", sv["i"]), ("pump_output", sv["p"])):
+        out = np.fromfile(tmp_path / f"{stem}_r.bin", "<f8") + 1j * np.fromfile(tmp_path / f"{stem}_i.bin", "<f8")
+        assert np.array_equal(out, np.asarray(ref).reshape(-1)), stem
