@@ -319,7 +319,7 @@ def sweep_section(weights):
         for key, use_mean in (("batched_fp32c_warm", False), ("batched_fp32c_mean_seed", True)):
             warm = sur.warm_fields_mixed([p[1:] for p in pts], [p[0] for p in pts], ap0, seed=11,
                                          use_mean=use_mean)
-            w = SW.run_batched(pts, precision=P.FP32C, warm=warm)
+            w = SW.run_warm(pts, warm, precision=P.FP32C)  # the cold start is the fallback
             out[key], _ = summarise(w)
             both = (cold_states == 1) & (w["states"] == 1)
             rel = np.abs(w["power"] - cold_power) / np.maximum(cold_power, 1e-300)
@@ -328,8 +328,11 @@ def sweep_section(weights):
                  "max": float(rel[both].max()) if both.any() else None}
             out[key]["same_finish_state_as_cold"] = int((w["states"] == cold_states).sum())
             out[key]["speedup_vs_per_instance"] = out[key]["points_per_s"] / (len(pts) / est)
+            out[key]["cold_fallbacks"] = int(w["fallback"].size)
             arms[key] = (w["states"], w["power"])
             w["batch"].close()
+            if w["fallback_batch"] is not None:
+                w["fallback_batch"].close()
     out["fp64_cold_check"] = fp64_steady_check(pts, arms)
     return out
 

@@ -38,10 +38,15 @@ def run(weights, count=32, N=4096, NZ=25, max_rt=20000, seed=7, precision=FP64):
     out = {"meta": meta, "points": pts}
     for name, kw in (("cold_a", dict(seed=1000)), ("cold_b", dict(seed=5000)), ("warm", dict(warm=warm)),
                      ("mean", dict(warm=mean))):
-        r = SW.run_batched(pts, NZ=NZ, max_rt=max_rt, precision=precision, **kw)
+        if "warm" in kw:  # warm starts fall back to cold_a's start when they diverge
+            r = SW.run_warm(pts, kw["warm"], seed=1000, NZ=NZ, max_rt=max_rt, precision=precision)
+        else:
+            r = SW.run_batched(pts, NZ=NZ, max_rt=max_rt, precision=precision, **kw)
         out[name] = dict(rts=r["rts"].tolist(), states=r["states"].tolist(), power=r["power"].tolist(),
-                         run_s=r["run_s"])
+                         run_s=r["run_s"], cold_fallbacks=int(len(r.get("fallback", ()))))
         r["batch"].close()
+        if r.get("fallback_batch") is not None:
+            r["fallback_batch"].close()
         print(name, "run_s", round(r["run_s"], 2), "states", np.bincount(r["states"] + 1, minlength=4).tolist(),
               flush=True)
     ca, cb, w, mn = (np.array(out[k]["power"]) for k in ("cold_a", "cold_b", "warm", "mean"))

@@ -180,6 +180,37 @@ def test_nonfinite_power_marks_instance_diverged(gpu):
     assert e.value.status == P.ERR_DIVERGED
 
 
+def test_diverged_warm_start_falls_back_to_the_cold_start(gpu):
+    """A warm start with a NaN sample makes its point diverge; run_warm runs
+    that point again from its cold-start noise and reports it, and the other
+    points keep their warm results.  The fallback's outcome equals the plain
+    cold run of the point."""
+    P = gpu
+    from cuopo_amd import sweep as SW
+    pts = [(1024, 3.0, 0.0, 20.0, 0.0, 0.95), (1024, 2.5, 0.3, 20.0, 0.0, 0.95), (2048, 3.0, -0.2, 20.0, 0.0, 0.95)]
+    warm = [(P.noise(7 + k, 0, 1e-6, q[0]), P.noise(7 + k, 1, 1e-6, q[0])) for k, q in enumerate(pts)]
+    bad = warm[1][0].copy()
+    bad[5] = np.nan
+    warm[1] = (bad, warm[1][1])
+    kw = dict(NZ=4, max_rt=300, precision=P.FP64, autotune=False)
+    plain = SW.run_batched(pts, warm=warm, **kw)
+    assert plain["states"][1] == -1 and -1 not in (plain["states"][0], plain["states"][2])
+    r = SW.run_warm(pts, warm, **kw)
+    cold = SW.run_batched(pts, **kw)
+    try:
+        assert list(r["fallback"]) == [1]
+        assert r["states"][1] == cold["states"][1] != -1
+        assert r["power"][1] == pytest.approx(cold["power"][1], rel=1e-12)
+        assert r["rts"][1] > cold["rts"][1] > 0 or cold["rts"][1] <= 0
+        for k in (0, 2):
+            assert r["states"][k] == plain["states"][k] and r["power"][k] == plain["power"][k]
+    finally:
+        for x in (plain, r, cold):
+            x["batch"].close()
+        if r["fallback_batch"] is not None:
+            r["fallback_batch"].close()
+
+
 def test_invalid_calls_fail_cleanly(gpu):
     P = gpu
     cfg = c1(N=1024, NZ=4, NRT=5)
