@@ -18,7 +18,7 @@ Format g_format = Format::Dat;
 bool write_bin(const std::string &path, const double *v, size_t n) {
   FILE *f = std::fopen(path.c_str(), "wb");
   if (!f) return false;
-  const bool ok = std::fwrite(v, sizeof(double), n, f) == n;
+  const bool ok = n == 0 || std::fwrite(v, sizeof(double), n, f) == n;  // v may be null when n == 0
   return std::fclose(f) == 0 && ok;
 }
 
