@@ -451,16 +451,21 @@ __global__ __launch_bounds__(row_max_threads(N2, E)) void row_stage(const OpoIns
   for (int m = 0; m < E; ++m) row[t + m * T2] = v[m];
 }
 
-// One thread per instance: reduce the column-tile partials in order, record
-// the power, advance the round-trip counter and, every check_every round
-// trips, test |P - P_prev| <= tol * P.
-__global__ void tick_kernel(OpoInst *__restrict__ insts, TickArgs a) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+// One wavefront per instance: reduce the column-tile partials in a fixed
+// order, record the power, advance the round-trip counter and, every
+// check_every round trips, test |P - P_prev| <= tol * P.
+__global__ __launch_bounds__(64) void tick_kernel(OpoInst *__restrict__ insts, TickArgs a) {
+  const int b = blockIdx.x;  // one wavefront per instance
   if (b >= a.count) return;
   OpoInst *I = insts + b;
-  if (I->active_v == 0) return;
+  if (I->active_v == 0) return;  // uniform over the wave
+  // the partials in a fixed order: lane l sums tiles l, l+64, ... in order,
+  // then a fixed shuffle tree over the lanes (deterministic, loads in parallel)
+  const int lane = threadIdx.x;
   double P = 0;
-  for (int j = 0; j < a.tiles; ++j) P += I->partials[j];
+  for (int j = lane; j < a.tiles; j += 64) P += I->partials[j];
+  for (int off = 32; off > 0; off >>= 1) P += __shfl_down(P, off, 64);
+  if (lane != 0) return;
   const int rt = *I->rt;
   if (I->power_hist && rt >= 0 && rt < I->hist_len) I->power_hist[rt] = P;
   const int done = rt + 1;
@@ -641,8 +646,7 @@ int launch_copy(const void *src, void *dst, long long n, void *stream, void *ev0
 }
 
 int launch_tick(OpoInst *insts, const TickArgs &a, void *stream) {
-  dim3 grid((a.count + 63) / 64);
-  hipLaunchKernelGGL(tick_kernel, grid, dim3(64), 0, (hipStream_t)stream, insts, a);
+  hipLaunchKernelGGL(tick_kernel, dim3(a.count), dim3(64), 0, (hipStream_t)stream, insts, a);
   return (int)hipGetLastError();
 }
 
