@@ -114,7 +114,10 @@ struct Engine {
   bool save_enabled = false;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
-  int graph_key = -1;
+  int graph_key = -1;  // 0: the captured graph matches the settings below; -1: rebuild
+  int graph_check_every = 0;
+  double graph_tol = 0;
+  bool graph_save = false;
   float last_ms = 0;
   int last_rt0 = 0, last_nrt = 0;
   int col_threads = 0, row_threads = 0;
@@ -601,8 +604,9 @@ int join_groups(Engine &E) {
 }
 
 int build_graph(Engine &E, int check_every, double tol) {
-  const int key = check_every * 1000003 + (int)(tol * 1e12) + (E.save_enabled ? 7 : 0);
-  if (E.gexec && E.graph_key == key) return OPO_OK;
+  if (E.gexec && E.graph_key == 0 && E.graph_check_every == check_every && E.graph_tol == tol &&
+      E.graph_save == E.save_enabled)
+    return OPO_OK;
   if (E.gexec) { (void)hipGraphExecDestroy(E.gexec); E.gexec = nullptr; }
   if (E.graph) { (void)hipGraphDestroy(E.graph); E.graph = nullptr; }
   HIPCHK(hipStreamBeginCapture(E.stream, hipStreamCaptureModeThreadLocal));
@@ -618,7 +622,10 @@ int build_graph(Engine &E, int check_every, double tol) {
   if (ee != hipSuccess) return fail(OPO_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
   E.graph = gr;
   HIPCHK(hipGraphInstantiate(&E.gexec, E.graph, nullptr, nullptr, 0));
-  E.graph_key = key;
+  E.graph_key = 0;
+  E.graph_check_every = check_every;
+  E.graph_tol = tol;
+  E.graph_save = E.save_enabled;
   return OPO_OK;
 }
 

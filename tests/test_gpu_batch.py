@@ -211,6 +211,38 @@ def test_diverged_warm_start_falls_back_to_the_cold_start(gpu):
             r["fallback_batch"].close()
 
 
+def test_graph_cache_follows_the_convergence_settings(gpu):
+    """The captured round-trip graph is reused only for the same convergence
+    settings: a batch run with tol = 0.02, then re-run with tol = 1e-6, matches
+    fresh batches run with each setting."""
+    P = gpu
+    cfgs = [c2(N=1024, NZ=4, NRT=400), c2(N=1024, NZ=4, NRT=400)]
+    cfgs[1].Nth = 1.5
+    made = [instance(P, c, 11 + k, 1e-3) for k, c in enumerate(cfgs)]
+
+    def fresh():
+        b = P.Batch([m[0] for m in made])
+        for k, (_, x) in enumerate(made):
+            b.upload(k, x["w"], x["T"], x["Ap_in"], x["Ap"], x["As"], x["Ai"])
+        return b
+
+    def result(b, tol):
+        rts, st = b.run_ex(400, check_every=50, tol=tol)
+        return rts.tolist(), st.tolist(), b.power().tolist()
+
+    a = fresh()
+    r1 = result(a, 0.02)
+    for k, (_, x) in enumerate(made):
+        a.upload(k, x["w"], x["T"], x["Ap_in"], x["Ap"], x["As"], x["Ai"])
+    r2 = result(a, 1e-6)
+    a.close()
+    for tol, r in ((0.02, r1), (1e-6, r2)):
+        b = fresh()
+        assert result(b, tol) == r, tol
+        b.close()
+    assert r1 != r2
+
+
 def test_invalid_calls_fail_cleanly(gpu):
     P = gpu
     cfg = c1(N=1024, NZ=4, NRT=5)
