@@ -191,6 +191,16 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def _f64(a, n):
+    """A real grid (w or T) of exactly n values as contiguous float64."""
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.size != n:
+        raise ValueError(f"expected a grid of {n} values, got {a.size}")
+    return a
+
+
 def _c128(a, n=None):
     if a is None:
         return None
@@ -303,8 +313,7 @@ class Solver:
             pass
 
     def upload(self, w, T, Ap_in, Ap, As, Ai=None):
-        w = np.ascontiguousarray(w, np.float64)
-        T = None if T is None else np.ascontiguousarray(T, np.float64)
+        w, T = _f64(w, self.N), _f64(T, self.N)
         Ap_in = _c128(Ap_in)
         Ap, As, Ai = _c128(Ap, self.N), _c128(As, self.N), _c128(Ai, self.N)
         _check(lib().opo_upload(self._h, _ptr(w), _ptr(T), _ptr(Ap_in), Ap_in.size, _ptr(Ap), _ptr(As), _ptr(Ai)))
@@ -421,8 +430,7 @@ class Batch:
 
     def upload(self, idx, w, T, Ap_in, Ap, As, Ai=None):
         N = self.params[idx].N
-        w = np.ascontiguousarray(w, np.float64)
-        T = None if T is None else np.ascontiguousarray(T, np.float64)
+        w, T = _f64(w, N), _f64(T, N)
         Ap_in = _c128(Ap_in)
         Ap, As, Ai = _c128(Ap, N), _c128(As, N), _c128(Ai, N)
         _check(lib().opo_batch_upload(self._h, idx, _ptr(w), _ptr(T), _ptr(Ap_in), Ap_in.size,

@@ -54,3 +54,18 @@ def test_invalid_params_rejected_without_gpu():
         assert "power of two" in str(e)
     else:
         raise AssertionError("N=1000 accepted")
+
+
+def test_binding_checks_grid_and_field_lengths():
+    """The binding refuses grids that are not exactly N long and fields
+    shorter than N before anything reaches the C ABI (which reads N values)."""
+    import numpy as np
+    import cuopo_amd as P
+    assert P._f64(np.arange(4), 4).dtype == np.float64
+    with pytest.raises(ValueError):
+        P._f64(np.zeros(3), 4)
+    with pytest.raises(ValueError):
+        P._f64(np.zeros(5), 4)
+    assert P._f64(None, 4) is None
+    with pytest.raises(ValueError):
+        P._c128(np.zeros(3), 4)
