@@ -60,6 +60,7 @@ def test_binding_checks_grid_and_field_lengths():
     """The binding refuses grids that are not exactly N long and fields
     shorter than N before anything reaches the C ABI (which reads N values)."""
     import numpy as np
+    import pytest
     import cuopo_amd as P
     assert P._f64(np.arange(4), 4).dtype == np.float64
     with pytest.raises(ValueError):
