@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <complex>
 #include <cstdlib>
@@ -954,6 +955,7 @@ int opo_boundary(opo_solver *s) {
 int opo_run(opo_solver *s, int rt0, int n_rt) {
   if (!s) return fail(OPO_ERR_INVALID, "null solver");
   if (rt0 < 0 || n_rt < 0) return fail(OPO_ERR_INVALID, "negative round-trip index");
+  if (n_rt > INT_MAX - rt0) return fail(OPO_ERR_INVALID, "rt0 + n_rt overflows int");
   HIPCHK(hipSetDevice(s->E.device));
   return engine_run(s->E, rt0, n_rt, 0, 0.0);
 }

@@ -101,3 +101,20 @@ def test_linear_step_absorption_only(gpu, N):
     for f, (g, a) in enumerate(zip(out, A)):
         alpha = (prm.alpha_crp, prm.alpha_crs, prm.alpha_cri)[f]
         assert rel_l2(g, a * np.exp(-0.5 * alpha * prm.dz)) < 1e-13
+
+
+def test_run_rejects_bad_round_trip_ranges(gpu):
+    """opo_run rejects a negative start or count and a range past INT_MAX before
+    any launch; the fields stay as uploaded."""
+    P = gpu
+    cfg = c1(N=1024, NZ=4, NRT=3)
+    p, T, w, Ap_in, Ap, As, Ai = inputs(cfg, seed=2)
+    s = solver(P, cfg)
+    s.upload(w, T, Ap_in, Ap, As, Ai)
+    before = s.download()
+    for rt0, n in [(-1, 1), (0, -1), (2**31 - 10, 11)]:
+        with pytest.raises(P.OpoError):
+            s.run(rt0, n)
+    for a, b in zip(before, s.download()):
+        if a is not None:
+            assert np.array_equal(a, b)
