@@ -12,7 +12,8 @@ Prints one JSON line: round trips/s, the roofline of the dominant kernel
 (per-launch kernel time measured with start/stop events on the solver's
 stream) and a CPU baseline (the FP64 oracle on one of this host's cores).
 Under "extra": the per-slice time inside the round-trip graph, the round-trip
-HBM fraction and FP64 flop rate, the N = 2^18 single instance and the 256
+HBM fraction and FP64 flop rate, one plain copy launch of each shape's stage
+bytes (the same-size ceiling), the N = 2^18 single instance and the 256
 instance FP64 batch, the 256-point sweep (batched FP32C against the plain
 per-instance FP64 solver, with the steady-power check against FP64 cold
 starts), and the CPU baselines of the other workloads (one core, all cores).
@@ -109,10 +110,12 @@ def c5_section(steps=20):
     col_us, row_us = s.time_stages(50)
     s.close()
     bz = 2 * 3 * d.N * 16
+    _, copy_us = P.copy_probe(3 * d.N * 16, 200)  # one copy launch of the same bytes
     rt_s = steps / wall
     return {"workload": "NS three-equation MgO:PPLN, N=2^18, 200 slices, EOM + GDD, FP64",
             "round_trips_per_s": rt_s, "col_stage_us": col_us, "row_stage_us": row_us,
             "col_frac": bz / (col_us * 1e-6) / HBM_PEAK, "row_frac": bz / (row_us * 1e-6) / HBM_PEAK,
+            "copy_same_bytes_us": copy_us, "copy_same_bytes_frac": bz / (copy_us * 1e-6) / HBM_PEAK,
             "round_trip_frac": b_rt(d.N, d.NZ, 3, 2, 16) * rt_s / HBM_PEAK, "tuning": tuning}
 
 
@@ -135,8 +138,10 @@ def c3_section(rts=10):
         cnt = sum(1 for p in pts if p[0] == N)
         c, r = b.time_stages(N, 20)
         bz = 2 * 3 * N * 16 * cnt
+        _, cp = P.copy_probe(3 * N * 16 * cnt, 50)  # one copy launch of the same bytes
         groups.append({"N": N, "count": cnt, "col_us": c, "row_us": r, "col_frac": bz / (c * 1e-6) / HBM_PEAK,
-                       "row_frac": bz / (r * 1e-6) / HBM_PEAK})
+                       "row_frac": bz / (r * 1e-6) / HBM_PEAK, "copy_same_bytes_us": cp,
+                       "copy_same_bytes_frac": bz / (cp * 1e-6) / HBM_PEAK})
     b.close()
     total = sum(b_rt(p[0], 100, 3, 2, 16) for p in pts)
     return {"workload": "256 CW three-equation MgO:PPLN instances, N in 2^12..2^15, 100 slices, FP64, one batch",
@@ -448,6 +453,9 @@ def main():
     achieved = Bz / (dom_us * 1e-6)
     traffic = load_traffic(f"c2_{dom}")
     copy_gbps, _ = P.copy_probe(256 << 20, 20)
+    # one copy launch that reads and writes the same Bz bytes, timed like the
+    # stages: what launch, fill and drain alone cost at this size
+    _, copy_bz_us = P.copy_probe(nf * N * sc, 200)
     out = {
         "metric": "round trips/s (plain FP64, one instance, cold start)",
         "value": rts,
@@ -475,6 +483,8 @@ def main():
                   "graph_slice_us": 1e3 * dev_ms / (a.steps * args.NZ),
                   "graph_slice_frac_of_hbm": 2 * Bz / (1e-3 * dev_ms / (a.steps * args.NZ)) / HBM_PEAK,
                   "copy_probe_GBps": copy_gbps, "tuning": tuning,
+                  "copy_same_bytes_us": copy_bz_us,
+                  "copy_same_bytes_frac_of_hbm": Bz / (copy_bz_us * 1e-6) / HBM_PEAK,
                   # whole round trip: B_rt x round trips/s against the HBM peak and the
                   # measured copy rate (a two-pass slice moves every field twice, so
                   # this fraction is at most half the per-launch one)
