@@ -66,32 +66,59 @@ def read_c(stem):
     return np.loadtxt(stem + "_r.dat") + 1j * np.loadtxt(stem + "_i.dat")
 
 
+# every drop-in build: (three equations, regime, crystal); the sPPLT builds use
+# the sweep script's grating period with a 1000 nm signal and the three-wave
+# phase-matching scan
+CLI_VARIANTS = {
+    "cuOPO_cw_ppln_2eq": (False, O.CW, O.PPLN),
+    "cuOPO_cw_spplt_2eq": (False, O.CW, O.SPPLT),
+    "cuOPO_cw_ppln_3eq": (True, O.CW, O.PPLN),
+    "cuOPO_cw_spplt_3eq": (True, O.CW, O.SPPLT),
+    "cuOPO_ns_ppln_3eq": (True, O.NS, O.PPLN),
+    "cuOPO_ns_spplt_3eq": (True, O.NS, O.SPPLT),
+}
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("variant", ["cuOPO_cw_ppln_3eq", "cuOPO_ns_ppln_3eq"])
+@pytest.mark.parametrize("variant", sorted(CLI_VARIANTS))
 def test_cli_outputs_match_oracle(gpu, tmp_path, variant):
-    ns = "_ns_" in variant
+    three, regime, crystal = CLI_VARIANTS[variant]
+    ns = regime == O.NS
+    spplt = crystal == O.SPPLT
     N, NZ, NRT = 1024, 4, (8 if ns else 70)
-    argv = ["532", "0", "6.97", "4", "98", "0.05", "100" if ns else "0", "0", "1" if ns else "0", "0.25", "4", "5"]
-    r = run([exe(variant)] + argv + [f"--N={N}", f"--nz={NZ}", f"--nrt={NRT}", "--seed=3", "--noise=1e-6"],
+    lam = "7.97" if spplt else "6.97"
+    argv = ["532", "0", lam, "4", "98", "0.05", "100" if ns else "0", "0", "1" if ns else "0", "0.25", "4", "5"]
+    extra = ["--ls=1.0", "--pm=3wave"] if spplt else []
+    r = run([exe(variant)] + argv + [f"--N={N}", f"--nz={NZ}", f"--nrt={NRT}", "--seed=3", "--noise=1e-6"] + extra,
             tmp_path)
     assert r.returncode == 0, r.stderr[-2000:]
-    cfg = O.Config(three_eqs=True, regime=O.NS if ns else O.CW, crystal=O.PPLN, N=N, NZ=NZ, NRT=NRT, lp_nm=532,
-                   Lambda=6.97, Nth=4, R_pct=98, delta=0.05, gdd_pct=100 if ns else 0, upm=1 if ns else 0,
+    cfg = O.Config(three_eqs=three, regime=regime, crystal=crystal, N=N, NZ=NZ, NRT=NRT, lp_nm=532,
+                   Lambda=float(lam), Nth=4, R_pct=98, delta=0.05, gdd_pct=100 if ns else 0, upm=1 if ns else 0,
                    md=0.25, fm=4, lcr_mm=5)
+    if spplt:
+        cfg.ls_um, cfg.pm_mode = 1.0, O.PM_THREE_WAVE
     p = O.setup(cfg)
     T, w = O.grids(p)
     Ap_in = O.pump_input(p)
     As = read_c(str(tmp_path / "signal_input"))
-    Ai = read_c(str(tmp_path / "idler_input"))
     assert rel_l2(As, O.noise(3, 0, 1e-6, N)) < 1e-15
     assert rel_l2(read_c(str(tmp_path / "pump_input")), Ap_in) < 1e-15
+    if three:
+        Ai = read_c(str(tmp_path / "idler_input"))
+        assert rel_l2(Ai, O.noise(3, 1, 1e-6, N)) < 1e-15
+    else:
+        Ai = None
+        assert not (tmp_path / "idler_input_r.dat").exists()
     Ap = Ap_in[:N].copy()
     sv, _ = O.run(p, w, T, Ap_in, Ap, As, Ai, 0, NRT, save=True)
     slots = NRT if ns else 64
-    for stem, key in (("signal_output", "s"), ("pump_output", "p"), ("idler_output", "i")):
+    stems = (("signal_output", "s"), ("pump_output", "p")) + ((("idler_output", "i"),) if three else ())
+    for stem, key in stems:
         out = read_c(str(tmp_path / stem))
         assert out.size == slots * N
         assert rel_l2(out, sv[key].reshape(-1)) < 1e-9, stem
+    if not three:
+        assert not (tmp_path / "idler_output_r.dat").exists()
     Tp = np.loadtxt(tmp_path / "Tp.dat")
     Tpo, Fpo = O.grids_long(p)
     assert Tp.size == p.SIZEL and np.allclose(Tp, Tpo, rtol=1e-15)
