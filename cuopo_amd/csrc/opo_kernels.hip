@@ -509,11 +509,17 @@ constexpr int kMaxDevices = 64;
 static int opt_in_big_lds(const void *kern, std::once_flag (&flags)[kMaxDevices]) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return (int)hipErrorInvalidDevice;
-  int rc = 0;
-  std::call_once(flags[dev], [&] {
-    rc = (int)hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  });
-  return rc;
+  // a failed opt-in leaves the flag unset (call_once does not complete when
+  // its callable throws), so the next launch on this device tries again
+  try {
+    std::call_once(flags[dev], [&] {
+      const hipError_t e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      if (e != hipSuccess) throw e;
+    });
+  } catch (hipError_t e) {
+    return (int)e;
+  }
+  return 0;
 }
 
 template <int NF> static bool col_is_mid(const ColArgs &a) {

@@ -9,6 +9,13 @@
  * (the cuOPO executable maps a non-zero status to the reference's
  * "Error: file:line" print and exit(1), src/headers/common.cuh:6-16).
  *
+ * Threads: each handle (solver, batch, MLP) owns its own HIP stream and
+ * device memory, and the library keeps no other mutable state, so distinct
+ * handles may be driven from different host threads at the same time.  Calls
+ * on one handle must not overlap.  opo_last_error() reports the calling
+ * thread's last failure.  (The reference runs one host thread on the legacy
+ * default stream of device 0, src/cuOPO.cu:78-82.)
+ *
  * Reference interfaces replaced (paths relative to the cuOPO source tree):
  *   opo_derive ........... setup block of main(), src/cuOPO.cu:92-269
  *   opo_grids/_long ...... linspace+fftshift grids, src/cuOPO.cu:170-189
