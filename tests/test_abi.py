@@ -56,6 +56,37 @@ def test_invalid_params_rejected_without_gpu():
         raise AssertionError("N=1000 accepted")
 
 
+def test_out_of_range_params_rejected_without_gpu():
+    """Grid sizes outside [256, 2^18], slices or round trips below one, an
+    unknown precision or regime, and a non-positive or non-finite step are
+    refused by validation, before any HIP call."""
+    import math
+    import pytest
+    import cuopo_amd as P
+    base = P.params_from(P.derive(P.Args(N=1024, NZ=4, NRT=2)))
+    bad = [("N", 128, "power of two"), ("N", 1 << 19, "power of two"), ("N", 3 << 10, "power of two"),
+           ("NZ", 0, "NZ and NRT"), ("NRT", 0, "NZ and NRT"), ("precision", 7, "precision"),
+           ("regime", 5, "regime"), ("Nrts", -1, "Nrts"), ("dz", 0.0, "dz"), ("dz", math.nan, "dz"),
+           ("dk", math.inf, "dk")]
+    for field, value, msg in bad:
+        p = P.Params.from_buffer_copy(base)
+        setattr(p, field, value)
+        with pytest.raises(P.OpoError) as e:
+            P.Solver(p)
+        assert e.value.status == P.ERR_INVALID and msg in str(e.value), (field, value, str(e.value))
+    # batches: at least one instance, and one shared equation count / regime / NZ / precision
+    with pytest.raises(P.OpoError) as e:
+        P.Batch([])
+    assert "count must be >= 1" in str(e.value)
+    for field, value in [("NZ", 5), ("three_eqs", 1 - base.three_eqs), ("regime", 1 - base.regime),
+                         ("precision", 1 - base.precision)]:
+        q = P.Params.from_buffer_copy(base)
+        setattr(q, field, value)
+        with pytest.raises(P.OpoError) as e:
+            P.Batch([base, q])
+        assert "must share" in str(e.value), field
+
+
 def test_binding_checks_grid_and_field_lengths():
     """The binding refuses grids that are not exactly N long and fields
     shorter than N before anything reaches the C ABI (which reads N values)."""
