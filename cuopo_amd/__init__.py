@@ -312,6 +312,12 @@ class Solver:
         except Exception:
             pass
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
     def upload(self, w, T, Ap_in, Ap, As, Ai=None):
         w, T = _f64(w, self.N), _f64(T, self.N)
         Ap_in = _c128(Ap_in)
@@ -427,6 +433,12 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def upload(self, idx, w, T, Ap_in, Ap, As, Ai=None):
         N = self.params[idx].N

@@ -63,6 +63,12 @@ class MLP:
         except Exception:
             pass
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
     def get_weights(self):
         w = np.empty(self.nparams, np.float32)
         _check(lib().opo_mlp_get_weights(self._h, _ptr(w), w.size))

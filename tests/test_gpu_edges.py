@@ -118,3 +118,23 @@ def test_run_rejects_bad_round_trip_ranges(gpu):
     for a, b in zip(before, s.download()):
         if a is not None:
             assert np.array_equal(a, b)
+
+
+def test_handles_close_on_leaving_a_with_block(gpu):
+    """Solver and Batch release their device memory when a with-block ends;
+    a closed handle reaches the C ABI as a null handle and is refused."""
+    P = gpu
+    cfg = c1(N=1024, NZ=4, NRT=3)
+    p, T, w, Ap_in, Ap, As, Ai = inputs(cfg, seed=4)
+    with solver(P, cfg) as s:
+        s.upload(w, T, Ap_in, Ap, As, Ai)
+        s.run(0, 2)
+    assert s._h is None
+    with pytest.raises(P.OpoError):
+        s.run(0, 1)
+    prm = P.params_from(P.derive(to_args(cfg)), 0)
+    with P.Batch([prm, prm]) as b:
+        for k in range(2):
+            b.upload(k, w, T, Ap_in, Ap, As, Ai)
+        b.run(2, check_every=0, tol=0.0)
+    assert b._h is None

@@ -66,3 +66,11 @@ def test_training_fits_a_teacher(gpu):
             last = net.train_step(X[s:s + 256], Y[s:s + 256], 1e-3)
     print("loss", first, "->", last)
     assert last < first / 20
+
+
+def test_mlp_closes_on_leaving_a_with_block(gpu):
+    from cuopo_amd.surrogate import MLP
+    with MLP(n_in=5, width=64, depth=1, n_out=80, max_batch=8, seed=3) as net:
+        y = net.forward(np.zeros((8, 5), np.float32))
+        assert np.isfinite(y).all()
+    assert net._h is None
