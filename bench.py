@@ -116,6 +116,8 @@ def c5_section(steps=20):
             "round_trips_per_s": rt_s, "col_stage_us": col_us, "row_stage_us": row_us,
             "col_frac": bz / (col_us * 1e-6) / HBM_PEAK, "row_frac": bz / (row_us * 1e-6) / HBM_PEAK,
             "copy_same_bytes_us": copy_us, "copy_same_bytes_frac": bz / (copy_us * 1e-6) / HBM_PEAK,
+            # HBM bytes per dispatch from the committed counter summary (FETCH_SIZE + WRITE_SIZE)
+            "col_traffic": load_traffic("c5_col_stage"), "row_traffic": load_traffic("c5_row_stage"),
             "round_trip_frac": b_rt(d.N, d.NZ, 3, 2, 16) * rt_s / HBM_PEAK, "tuning": tuning}
 
 
