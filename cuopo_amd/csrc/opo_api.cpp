@@ -118,6 +118,7 @@ struct Engine {
   int graph_key = -1;  // 0: the captured graph matches the settings below; -1: rebuild
   int graph_check_every = 0;
   double graph_tol = 0;
+  double graph_off_ratio = 0;
   bool graph_save = false;
   float last_ms = 0;
   int last_rt0 = 0, last_nrt = 0;
@@ -605,8 +606,11 @@ int join_groups(Engine &E) {
 }
 
 int build_graph(Engine &E, int check_every, double tol) {
+  // the captured kernel arguments carry check_every (column stages, tick),
+  // tol and off_ratio (tick); a change of these or of the save switch means a
+  // new capture
   if (E.gexec && E.graph_key == 0 && E.graph_check_every == check_every && E.graph_tol == tol &&
-      E.graph_save == E.save_enabled)
+      E.graph_off_ratio == E.off_ratio && E.graph_save == E.save_enabled)
     return OPO_OK;
   if (E.gexec) { (void)hipGraphExecDestroy(E.gexec); E.gexec = nullptr; }
   if (E.graph) { (void)hipGraphDestroy(E.graph); E.graph = nullptr; }
@@ -626,6 +630,7 @@ int build_graph(Engine &E, int check_every, double tol) {
   E.graph_key = 0;
   E.graph_check_every = check_every;
   E.graph_tol = tol;
+  E.graph_off_ratio = E.off_ratio;
   E.graph_save = E.save_enabled;
   return OPO_OK;
 }
@@ -1166,8 +1171,7 @@ int opo_batch_run_ex(opo_batch *b, int max_rt, int check_every, double tol, doub
   Engine &E = b->E;
   HIPCHK(hipSetDevice(E.device));
   if (tol <= 0) check_every = 0;
-  E.off_ratio = off_ratio;
-  E.graph_key = -1;
+  E.off_ratio = off_ratio;  // part of the round-trip graph's cache key
   RCHK(engine_run(E, 0, max_rt, check_every, tol));
   int rc = OPO_OK;
   for (auto &g : E.groups) {

@@ -243,6 +243,44 @@ def test_graph_cache_follows_the_convergence_settings(gpu):
     assert r1 != r2
 
 
+def test_graph_reuse_across_runs_and_off_ratio_changes(gpu, monkeypatch):
+    """A batch re-run without new uploads replays the round-trip graph it
+    captured; a different off_ratio (an argument of the captured tick kernel)
+    makes it capture again.  Three runs in a row (off_ratio 1e-3, 1e-3, then
+    1e-12) equal the same three runs of a batch that launches every kernel
+    eagerly; the second run declares both instances off, the third (stricter
+    collapse) runs them to the cap."""
+    P = gpu
+    cfgs = [c2(N=1024, NZ=4, NRT=1500), c2(N=1024, NZ=4, NRT=1500)]
+    cfgs[0].Nth, cfgs[1].Nth = 0.5, 0.7  # below threshold: both decay and are declared off
+    made = [instance(P, c, 21 + k, 1e-3) for k, c in enumerate(cfgs)]
+
+    def fresh():
+        b = P.Batch([m[0] for m in made])
+        for k, (_, x) in enumerate(made):
+            b.upload(k, x["w"], x["T"], x["Ap_in"], x["Ap"], x["As"], x["Ai"])
+        return b
+
+    def runs(b):
+        out = []
+        for off in (1e-3, 1e-3, 1e-12):
+            rts, st = b.run_ex(1500, check_every=50, tol=1e-6, off_ratio=off)
+            out.append((rts.tolist(), st.tolist(), b.power().tolist()))
+        return out
+
+    g = fresh()
+    with_graph = runs(g)
+    g.close()
+    monkeypatch.setenv("OPO_USE_GRAPH", "0")  # read when the batch is created
+    e = fresh()
+    eager = runs(e)
+    e.close()
+    print(with_graph)
+    assert with_graph == eager
+    assert with_graph[1][1] == [2, 2]  # the replayed graph declares both off under 1e-3
+    assert with_graph[2][1] == [0, 0]  # the new capture (1e-12) runs them to the cap
+
+
 def test_invalid_calls_fail_cleanly(gpu):
     P = gpu
     cfg = c1(N=1024, NZ=4, NRT=5)
