@@ -25,5 +25,5 @@ def gpu():
     import cuopo_amd as P
     P.lib()
     if P.device_count() < 1:
-        pytest.fail("no HIP device visible: GPU tests must run on a GPU box")
+        pytest.fail("no HIP device visible: the tests marked gpu need an MI355X (gfx950)")
     return P
