@@ -156,6 +156,22 @@ def test_cli_fp32c_against_fp64_oracle(gpu, tmp_path):
 
 
 @pytest.mark.gpu
+def test_out_directory_receives_every_file(gpu, tmp_path):
+    """--out=<dir> puts every output in that folder and nothing in the
+    working directory (the reference writes to the working directory)."""
+    sub = tmp_path / "run1"
+    sub.mkdir()
+    argv = ["532", "0", "6.97", "4", "98", "0.05", "0", "0", "0", "0.25", "4", "5",
+            "--N=256", "--nz=2", "--nrt=70", "--seed=3", "--noise=1e-6", f"--out={sub}"]
+    r = run([exe("cuOPO_cw_ppln_3eq")] + argv, tmp_path)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert sorted(f.name for f in tmp_path.iterdir()) == ["run1"]
+    names = sorted(os.listdir(sub))
+    assert len(names) == 15 and all(n.endswith(".dat") for n in names)
+    assert {"T.dat", "Tp.dat", "freq.dat", "signal_output_r.dat", "idler_input_i.dat"} <= set(names)
+
+
+@pytest.mark.gpu
 def test_binary_side_cars_equal_the_text_files(gpu, tmp_path):
     """--format=both writes X.bin (raw float64) beside every X.dat with the
     same values; --format=bin writes only the .bin files, identical bytes for
