@@ -121,13 +121,15 @@ def c5_section(steps=20):
             "round_trip_frac": b_rt(d.N, d.NZ, 3, 2, 16) * rt_s / HBM_PEAK, "tuning": tuning}
 
 
-def c3_section(rts=10):
-    """The 256-point sweep family as one FP64 batch, fixed round-trip count."""
+def c3_section(rts=10, fp32c=False):
+    """The 256-point sweep family as one batch (FP64, or FP32C as the sweep
+    runs it), fixed round-trip count."""
     import numpy as np
     import cuopo_amd as P
     from cuopo_amd import sweep as SW
     pts = SW.sweep_points(256, seed=3)
-    b, made = SW.make_batch(pts, max_rt=rts + 10, precision=P.FP64)
+    sc = 8 if fp32c else 16
+    b, made = SW.make_batch(pts, max_rt=rts + 10, precision=P.FP32C if fp32c else P.FP64)
     log("c3: tuning")
     tuning = tune(b)
     log("c3: running")
@@ -139,14 +141,15 @@ def c3_section(rts=10):
     for N in SW.SWEEP_N:
         cnt = sum(1 for p in pts if p[0] == N)
         c, r = b.time_stages(N, 20)
-        bz = 2 * 3 * N * 16 * cnt
-        _, cp = P.copy_probe(3 * N * 16 * cnt, 50)  # one copy launch of the same bytes
+        bz = 2 * 3 * N * sc * cnt
+        _, cp = P.copy_probe(3 * N * sc * cnt, 50)  # one copy launch of the same bytes
         groups.append({"N": N, "count": cnt, "col_us": c, "row_us": r, "col_frac": bz / (c * 1e-6) / HBM_PEAK,
                        "row_frac": bz / (r * 1e-6) / HBM_PEAK, "copy_same_bytes_us": cp,
                        "copy_same_bytes_frac": bz / (cp * 1e-6) / HBM_PEAK})
     b.close()
-    total = sum(b_rt(p[0], 100, 3, 2, 16) for p in pts)
-    return {"workload": "256 CW three-equation MgO:PPLN instances, N in 2^12..2^15, 100 slices, FP64, one batch",
+    total = sum(b_rt(p[0], 100, 3, 2, sc) for p in pts)
+    prec = "FP32C" if fp32c else "FP64"
+    return {"workload": f"256 CW three-equation MgO:PPLN instances, N in 2^12..2^15, 100 slices, {prec}, one batch",
             "round_trips_per_s_per_instance": rts / wall, "round_trip_frac": total * rts / wall / HBM_PEAK,
             "groups": groups, "tuning": tuning}
 
@@ -401,9 +404,9 @@ def main():
     ap.add_argument("--tuning", default=None,
                     help="col_threads,row_threads,xcd_pack,phase_mode[,col_spt,row_spt] applied instead of autotuning "
                          "(profiling runs: the profiler perturbs the autotuner's timings)")
-    ap.add_argument("--only", choices=["c5", "c3"], default=None,
-                    help="run only the N=2^18 (c5) or the batched FP64 (c3) section and print its JSON "
-                         "(for profiling those shapes)")
+    ap.add_argument("--only", choices=["c5", "c3", "c3f"], default=None,
+                    help="run only the N=2^18 (c5), the batched FP64 (c3) or the batched FP32C (c3f, the "
+                         "sweep's arithmetic) section and print its JSON (for profiling those shapes)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "cuopo_amd", "data", "surrogate.npz"))
     a = ap.parse_args()
     if a.gpus != 1:
@@ -421,7 +424,7 @@ def main():
         if len(TUNING) not in (4, 6):
             raise SystemExit("--tuning wants col_threads,row_threads,xcd_pack,phase_mode[,col_spt,row_spt]")
     if a.only:
-        print(json.dumps(c5_section() if a.only == "c5" else c3_section()))
+        print(json.dumps(c5_section() if a.only == "c5" else c3_section(fp32c=a.only == "c3f")))
         return
 
     args = workload_c2(NRT=a.warmup + a.steps + 1)
