@@ -66,14 +66,16 @@ def read_c(stem):
     return np.loadtxt(stem + "_r.dat") + 1j * np.loadtxt(stem + "_i.dat")
 
 
-# every drop-in build: (three equations, regime, crystal); the sPPLT builds use
-# the sweep script's grating period with a 1000 nm signal and the three-wave
-# phase-matching scan
+# every drop-in build, one per switch combination: (three equations, regime,
+# crystal); the sPPLT builds use the sweep script's grating period with a
+# 1000 nm signal and the three-wave phase-matching scan
 CLI_VARIANTS = {
     "cuOPO_cw_ppln_2eq": (False, O.CW, O.PPLN),
     "cuOPO_cw_spplt_2eq": (False, O.CW, O.SPPLT),
     "cuOPO_cw_ppln_3eq": (True, O.CW, O.PPLN),
     "cuOPO_cw_spplt_3eq": (True, O.CW, O.SPPLT),
+    "cuOPO_ns_ppln_2eq": (False, O.NS, O.PPLN),
+    "cuOPO_ns_spplt_2eq": (False, O.NS, O.SPPLT),
     "cuOPO_ns_ppln_3eq": (True, O.NS, O.PPLN),
     "cuOPO_ns_spplt_3eq": (True, O.NS, O.SPPLT),
 }

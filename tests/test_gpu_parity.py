@@ -132,7 +132,7 @@ def test_crystal_pass(gpu, which):
         assert rel_l2(gi, Ai) < 1e-11
 
 
-@pytest.mark.parametrize("which", ["c1", "c2", "c5"])
+@pytest.mark.parametrize("which", ["c1", "c2", "c5", "c5_2eq"])
 def test_one_round_trip(gpu, which):
     if which == "c1":
         cfg = c1(N=4096, NZ=32, NRT=2000)
@@ -140,6 +140,7 @@ def test_one_round_trip(gpu, which):
         cfg = c2(N=16384, NZ=100, NRT=5000)
     else:
         cfg = c5(N=4096, NZ=16, NRT=10)
+        cfg.three_eqs = which == "c5"  # NS with two equations: the switches are independent
     p, T, w, Ap_in, Ap, As, Ai = inputs(cfg, seed=2)
     # start from a grown signal so the nonlinear terms matter
     As = As * 1e19
