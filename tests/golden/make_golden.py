@@ -2,7 +2,7 @@
 
 Each fixture holds the reference argv/switches, the seeded initial fields and
 the fields after NRT round trips (plus the power history).  Run from the
-repository root after `make -C oracle`:  python tests/golden/make_golden.py
+repository root after `make -C oracle`:  python tests/golden/make_golden.py [name ...]
 """
 import json
 import os
@@ -16,10 +16,19 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 from oracle import oracle as O  # noqa: E402
 from tests.helpers import c1, c2, c5, inputs  # noqa: E402
 
+def ns_spplt_2eq(N=1024, NZ=8, NRT=5):
+    """NS, sPPLT, two equations (degenerate signal, the reference's phase-matching
+    scan), EOM and GDD: the -DNS_OPO -DSPPLT build without -DTHREE_EQS."""
+    import math
+    return O.Config(three_eqs=False, regime=O.NS, crystal=O.SPPLT, N=N, NZ=NZ, NRT=NRT, lp_nm=532, Lambda=7.97,
+                    Nth=4.0, R_pct=98, gdd_pct=100, upm=1, md=0.8 / math.pi, fm=4, lcr_mm=5)
+
+
 CASES = {
     "cw_ppln_2eq": (lambda: c1(N=1024, NZ=8, NRT=5), 11, 1e-3),
     "cw_spplt_3eq": (lambda: c2(N=1024, NZ=8, NRT=5), 12, 1e-3),
     "ns_ppln_3eq_eom_gdd": (lambda: c5(N=1024, NZ=8, NRT=5), 13, 1e-3),
+    "ns_spplt_2eq_eom_gdd": (ns_spplt_2eq, 14, 1e-3),
 }
 
 
@@ -37,6 +46,8 @@ def make(name):
 
 
 if __name__ == "__main__":
-    for n in CASES:
+    # names on the command line (default: every case); an existing fixture is
+    # rewritten only when named, since the compressed file is not byte-stable
+    for n in sys.argv[1:] or CASES:
         make(n)
         print("wrote", n)
