@@ -13,7 +13,8 @@ Prints one JSON line: round trips/s, the roofline of the dominant kernel
 stream) and a CPU baseline (the FP64 oracle on one of this host's cores).
 Under "extra": the per-slice time inside the round-trip graph, the round-trip
 HBM fraction and FP64 flop rate, one plain copy launch of each shape's stage
-bytes (the same-size ceiling), the N = 2^18 single instance and the 256
+bytes (the same-size ceiling), the upload/download times and the rate with
+them included (the C ABI takes host buffers), the N = 2^18 single instance and the 256
 instance FP64 batch, the 256-point sweep (batched FP32C against the plain
 per-instance FP64 solver, with the steady-power check against FP64 cold
 starts), and the CPU baselines of the other workloads (one core, all cores).
@@ -446,9 +447,16 @@ def main():
     wall = time.perf_counter() - t0
     dev_ms = s.last_run_ms()
     rts = a.steps / wall
+    t1 = time.perf_counter()
     Ap, As, Ai = s.download()
+    down_s = time.perf_counter() - t1
     if not (np.isfinite(As).all() and np.isfinite(Ap).all()):
         raise SystemExit("non-finite fields after the timed run")
+    # the C ABI takes host buffers: one upload (host-side phase tables + H2D)
+    # and one download (D2H) around the timed run give the PCIe-inclusive rate
+    t1 = time.perf_counter()
+    s.upload(inst["w"], inst["T"], inst["Ap_in"], inst["Ap"], inst["As"], inst["Ai"])
+    up_s = time.perf_counter() - t1
 
     # roofline of the dominant kernel: per-launch times of the slice's two stages
     col_us, row_us = s.time_stages(200)
@@ -489,6 +497,8 @@ def main():
                   "graph_slice_frac_of_hbm": 2 * Bz / (1e-3 * dev_ms / (a.steps * args.NZ)) / HBM_PEAK,
                   "copy_probe_GBps": copy_gbps, "tuning": tuning,
                   "copy_same_bytes_us": copy_bz_us,
+                  "upload_ms": 1e3 * up_s, "download_ms": 1e3 * down_s,
+                  "round_trips_per_s_incl_pcie": a.steps / (wall + up_s + down_s),
                   "copy_same_bytes_frac_of_hbm": Bz / (copy_bz_us * 1e-6) / HBM_PEAK,
                   # whole round trip: B_rt x round trips/s against the HBM peak and the
                   # measured copy rate (a two-pass slice moves every field twice, so
