@@ -220,11 +220,26 @@ int main(int argc, char *argv[]) {
   opo_cli::write_real(out + "/Tp.dat", Tp.data(), d.SIZEL);
   opo_cli::write_real(out + "/freq.dat", Fp.data(), d.SIZEL);
   opo_cli::write_real(out + "/T.dat", T.data(), N);
+  // one field at a time, in the order written: each field's device-to-host
+  // copy overlaps the formatting of the one before
   std::vector<double> sp(2 * d.SIZEL), ss(2 * d.SIZEL), si(d.three_eqs ? 2 * d.SIZEL : 0);
-  CHECK(opo_download_saved(s, sp.data(), ss.data(), d.three_eqs ? si.data() : nullptr));
-  opo_cli::write_complex(out + "/signal_output", ss.data(), d.SIZEL);
-  opo_cli::write_complex(out + "/pump_output", sp.data(), d.SIZEL);
-  if (d.three_eqs) opo_cli::write_complex(out + "/idler_output", si.data(), d.SIZEL);
+  const char *stems[3] = {"/signal_output", "/pump_output", "/idler_output"};
+  double *bufs[3] = {ss.data(), sp.data(), si.data()};
+  std::string fetch_err;
+  const int rc = opo_cli::fetch_and_write(
+      d.three_eqs ? 3 : 2,
+      [&](int k) {
+        const int st = opo_download_saved(s, k == 1 ? bufs[1] : nullptr, k == 0 ? bufs[0] : nullptr,
+                                          k == 2 ? bufs[2] : nullptr);
+        if (st != OPO_OK) fetch_err = opo_last_error();
+        return st;
+      },
+      [&](int k) { opo_cli::write_complex(out + stems[k], bufs[k], d.SIZEL); });
+  if (rc != OPO_OK) {
+    std::fprintf(stderr, "Error: %s:%d, ", __FILE__, __LINE__);
+    std::fprintf(stderr, "code: %d, reason: %s (%s)\n", rc, opo_strerror(rc), fetch_err.c_str());
+    return 1;
+  }
   CHECK(opo_destroy(s));
 
   const double iElaps = seconds() - iStart;

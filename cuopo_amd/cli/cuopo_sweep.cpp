@@ -297,11 +297,26 @@ int main(int argc, char *argv[]) {
       opo_cli::write_real(dir + "/Tp.dat", P.Tp.data(), d.SIZEL);
       opo_cli::write_real(dir + "/freq.dat", P.Fp.data(), d.SIZEL);
       opo_cli::write_real(dir + "/T.dat", P.T.data(), d.N);
+      // one field at a time: each field's device-to-host copy overlaps the
+      // formatting of the one before
       std::vector<double> sp(2 * d.SIZEL), ss(2 * d.SIZEL), si(three ? 2 * d.SIZEL : 0);
-      CHECK(opo_batch_download_saved(b, (int)(k - c0), sp.data(), ss.data(), three ? si.data() : nullptr));
-      opo_cli::write_complex(dir + "/signal_output", ss.data(), d.SIZEL);
-      opo_cli::write_complex(dir + "/pump_output", sp.data(), d.SIZEL);
-      if (three) opo_cli::write_complex(dir + "/idler_output", si.data(), d.SIZEL);
+      const char *stems[3] = {"/signal_output", "/pump_output", "/idler_output"};
+      double *bufs[3] = {ss.data(), sp.data(), si.data()};
+      std::string fetch_err;
+      const int rc = opo_cli::fetch_and_write(
+          three ? 3 : 2,
+          [&](int f) {
+            const int st = opo_batch_download_saved(b, (int)(k - c0), f == 1 ? bufs[1] : nullptr,
+                                                    f == 0 ? bufs[0] : nullptr, f == 2 ? bufs[2] : nullptr);
+            if (st != OPO_OK) fetch_err = opo_last_error();
+            return st;
+          },
+          [&](int f) { opo_cli::write_complex(dir + stems[f], bufs[f], d.SIZEL); });
+      if (rc != OPO_OK) {
+        std::fprintf(stderr, "Error: %s:%d, ", __FILE__, __LINE__);
+        std::fprintf(stderr, "code: %d, reason: %s (%s)\n", rc, opo_strerror(rc), fetch_err.c_str());
+        return 1;
+      }
       std::cout << "  " << pts[k].folder << std::endl;
     }
     CHECK(opo_batch_destroy(b));

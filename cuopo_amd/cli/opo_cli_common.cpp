@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <ctime>
+#include <future>
 #include <string>
 #include <thread>
 #include <vector>
@@ -24,6 +25,24 @@ bool write_bin(const std::string &path, const double *v, size_t n) {
 
 bool write_dat(const std::string &path, const double *v, size_t n, int threads);
 }  // namespace
+
+int fetch_and_write(int count, const std::function<int(int)> &fetch, const std::function<void(int)> &write) {
+  std::vector<std::promise<int>> done(count > 0 ? count : 0);
+  std::thread helper([&] {
+    int rc = OPO_OK;
+    for (int k = 0; k < count; ++k) {
+      if (rc == OPO_OK) rc = fetch(k);
+      done[k].set_value(rc);  // after a failure every later item reports it too
+    }
+  });
+  int rc = OPO_OK;
+  for (int k = 0; k < count && rc == OPO_OK; ++k) {
+    rc = done[k].get_future().get();
+    if (rc == OPO_OK) write(k);
+  }
+  helper.join();
+  return rc;
+}
 
 bool set_format(const std::string &name) {
   if (name == "dat") g_format = Format::Dat;

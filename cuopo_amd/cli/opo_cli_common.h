@@ -4,6 +4,7 @@
 #pragma once
 
 #include <cstddef>
+#include <functional>
 #include <ostream>
 #include <string>
 
@@ -26,6 +27,15 @@ bool write_real(const std::string &path, const double *v, size_t n, int threads 
 // Interleaved complex -> stem_r.dat / stem_i.dat (SaveVectorComplex,
 // files.cuh:43-62).
 bool write_complex(const std::string &stem, const double *v, size_t n, int threads = 0);
+
+// Device-to-host copies overlapped with the writes: fetch(0), fetch(1), ...
+// run in order on a helper thread, and write(k) runs on the calling thread as
+// soon as fetch(k) has returned OPO_OK, so each copy overlaps the formatting
+// of the items before it.  Returns OPO_OK, or the first failing fetch's status
+// (the items from that one on are not written).  opo_last_error() is per
+// thread, so a fetch that fails keeps its own message for the caller; it is
+// readable once this returns.
+int fetch_and_write(int count, const std::function<int(int)> &fetch, const std::function<void(int)> &write);
 
 // "Welcome", date, regime and equation lines of src/cuOPO.cu:52-64, 75-88.
 void print_banner(std::ostream &o, int regime, int three_eqs);

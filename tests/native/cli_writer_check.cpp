@@ -1,6 +1,6 @@
 // Sanitizer driver for the CLI's host code (test infrastructure): the .dat
 // writer in every output format, on sizes around its 64 Ki-value blocks, and
-// the parameter dump, built with -fsanitize=address,undefined by
+// the parameter dump, and the overlapped fetch/write helper, built with -fsanitize=address,undefined by
 // tests/native/Makefile.  Reads every file back and exits 0 when all values
 // round-trip and no sanitizer fired.
 #include <cmath>
@@ -60,5 +60,19 @@ int main(int argc, char **argv) {
   opo_cli::print_banner(o, OPO_CW, 1);
   opo_cli::print_params(o, d, a, 50.0, 0.0, 1.0);
   ok = ok && o.str().find("Number of round trips") != std::string::npos;
+  // fetch_and_write: every item written once, in order, each after its own
+  // fetch; a failing fetch stops the writes there and its status comes back
+  for (int fail_at : {-1, 0, 2}) {
+    std::vector<int> fetched(4, 0), written;
+    const int rc = opo_cli::fetch_and_write(
+        4, [&](int k) { fetched[k] = 1; return k == fail_at ? OPO_ERR_HIP : OPO_OK; },
+        [&](int k) { if (fetched[k]) written.push_back(k); });
+    const size_t expect = fail_at < 0 ? 4 : (size_t)fail_at;
+    bool c = rc == (fail_at < 0 ? OPO_OK : OPO_ERR_HIP) && written.size() == expect;
+    for (size_t i = 0; i < written.size(); ++i) c = c && written[i] == (int)i;
+    for (int k = fail_at + 1; fail_at >= 0 && k < 4; ++k) c = c && !fetched[k];
+    std::printf("fetch_and_write fail_at=%d: %s\n", fail_at, c ? "ok" : "bad");
+    ok = ok && c;
+  }
   return ok ? 0 : 1;
 }
