@@ -184,3 +184,28 @@ def test_noise_generator(oracle_lib):
                                      rel=1e-15)
     # different streams differ
     assert not np.allclose(O.noise(42, 0, 1.0, 16), O.noise(42, 1, 1.0, 16))
+
+
+@pytest.mark.parametrize("cr", CRYSTALS)
+def test_product_material_abi_equals_oracle(oracle_lib, cr):
+    """The exported material / phase-matching C ABI (opo_n, opo_group_velocity,
+    opo_gvd, opo_find_pm_temp) equals the oracle's restatement over a grid of
+    wavelengths and temperatures, and its group velocity and GVD follow from
+    central differences of its own opo_n (host code: no GPU needed)."""
+    import cuopo_amd as P
+    L_, O_ = P.lib(), oracle_lib.lib()
+    C = 299792458e6 / 1e12
+    h = 1e-4
+    for L in (0.532, 0.8, 1.0, 1.064, 1.55, 2.2, 3.1):
+        for T in (15.0, 37.5, 70.0, 120.0):
+            n = L_.opo_n(cr, L, T)
+            assert n == pytest.approx(O_.orc_n(cr, L, T), rel=1e-14)
+            assert L_.opo_group_velocity(cr, L, T) == pytest.approx(O_.orc_GV(cr, L, T), rel=1e-13)
+            assert L_.opo_gvd(cr, L, T) == pytest.approx(O_.orc_GVD(cr, L, T), rel=1e-12)
+            fd1 = (L_.opo_n(cr, L + h, T) - L_.opo_n(cr, L - h, T)) / (2 * h)
+            fd2 = (L_.opo_n(cr, L + h, T) - 2 * n + L_.opo_n(cr, L - h, T)) / h ** 2
+            assert L_.opo_group_velocity(cr, L, T) == pytest.approx(C / (n - L * fd1), rel=1e-7)
+            assert L_.opo_gvd(cr, L, T) == pytest.approx(L ** 3 * fd2 / (2 * math.pi * C ** 2), rel=1e-4)
+    for lp, ls, lam, mode in ((0.532, 1.064, 6.97, 0), (1.55, 3.1, 35.01, 0), (0.532, 1.0, 7.97, 1),
+                              (0.532, 1.064, 7.97, 0)):
+        assert P.find_pm_temp(cr, lp, ls, lam, pm_mode=mode) == O_.orc_find_pm_temp(cr, lp, ls, lam, 10, 150, mode)
