@@ -108,3 +108,25 @@ def test_set_tuning_rejects_bad_values(gpu):
     with pytest.raises(P.OpoError):
         s.set_tuning(row_threads=128, row_spt=16)
     assert s.get_tuning()["row_spt"] == 4
+
+
+def test_xcd_pack_entry_point_and_timers(gpu):
+    """opo_set_xcd_pack changes where blocks run, not the result; the stage and
+    run timers report positive times, the run's no shorter than its slices."""
+    P = gpu
+    cfg = c2(N=16384, NZ=6, NRT=5)
+    ref, _ = solver_for(P, cfg)
+    ref.run(0, 2)
+    s, _ = solver_for(P, cfg)
+    s.set_xcd_pack(True)
+    assert s.get_tuning()["xcd_pack"] == 1
+    s.run(0, 2)
+    for a, b in zip(s.download(), ref.download()):
+        assert rel_l2(a, b) < 1e-13
+    col_us, row_us = s.time_stages(5)
+    assert col_us > 0 and row_us > 0
+    s.run(0, 4)
+    ms = s.last_run_ms()
+    assert ms > 0 and ms * 1e3 >= 4 * cfg.NZ * min(col_us, row_us) * 0.5
+    s.close()
+    ref.close()
