@@ -361,7 +361,7 @@ class Solver:
         return out
 
     def power_history(self, n):
-        out = np.empty(n)
+        out = np.zeros(n)
         _check(lib().opo_power_history(self._h, _ptr(out), n))
         return out
 

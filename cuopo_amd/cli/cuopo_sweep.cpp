@@ -206,7 +206,6 @@ int main(int argc, char *argv[]) {
   if (reg != "cw" && reg != "ns") die("REG must be cw or ns");
   if (eqs != "2" && eqs != "3") die("EQS must be 2 or 3");
   const int regime = reg == "ns" ? OPO_NS : OPO_CW, three = eqs == "3";
-  if (regime == OPO_NS && !three) die("the nanosecond regime uses three equations (src/cuOPO.cu:65-70)");
   const std::vector<Point> pts = expand(spec);
   std::cout << "cuopo_sweep: " << pts.size() << " points, " << (three ? 3 : 2) << " equations, "
             << (regime == OPO_NS ? "ns" : "cw") << ", N = " << fl.N << ", NZ = " << fl.NZ << ", NRT = " << fl.NRT
@@ -221,11 +220,34 @@ int main(int argc, char *argv[]) {
   }
   make_dir(fl.out);
 
+  // the reference's positional arguments of point k (src/cuOPO.cu:92-267)
+  auto args_of = [&](size_t k) {
+    const auto &v = pts[k].argv;
+    opo_args a{};
+    a.three_eqs = three; a.regime = regime;
+    a.crystal = pts[k].crystal == "PPLN" ? OPO_PPLN : OPO_SPPLT;
+    a.N = fl.N; a.NZ = fl.NZ; a.NRT = fl.NRT;
+    a.lp_nm = std::atof(v[0].c_str()); a.temp_arg = std::atof(v[1].c_str()); a.Lambda = std::atof(v[2].c_str());
+    a.Nth = std::atof(v[3].c_str()); a.R_pct = std::atof(v[4].c_str()); a.delta = std::atof(v[5].c_str());
+    a.gdd_pct = std::atof(v[6].c_str()); a.tod_pct = std::atof(v[7].c_str()); a.upm = std::atoi(v[8].c_str());
+    a.md = std::atof(v[9].c_str()); a.fm = std::atof(v[10].c_str()); a.lcr_mm = std::atof(v[11].c_str());
+    a.ls_um = fl.ls_um; a.pm_mode = fl.pm_mode;
+    return a;
+  };
+
   // per-point device bytes: state + tables (~4 NF N) + pump (N or N*NRT) +
-  // save slots (NF * slots * N), complex of 16 or 8 bytes
+  // save slots (NF * slots * N, slots = the derived Nrts for CW, NRT for
+  // NS), complex of 16 or 8 bytes; the largest point sizes the chunks
   const double es = fl.precision == OPO_FP32C ? 8 : 16;
-  const double slots = regime == OPO_CW ? 64 : fl.NRT;
-  const double per_point = es * fl.N * ((three ? 3 : 2) * (4 + slots) + (regime == OPO_CW ? 1 : fl.NRT));
+  double per_point = 1.0;
+  for (size_t k = 0; k < pts.size(); ++k) {
+    opo_args a = args_of(k);
+    opo_derived d;
+    CHECK(opo_derive(&a, &d));
+    const double slots = regime == OPO_CW ? d.Nrts : d.NRT;
+    const double pump = regime == OPO_CW ? 1.0 : d.NRT;
+    per_point = std::max(per_point, es * d.N * ((three ? 3 : 2) * (4 + slots) + pump));
+  }
   const int by_mem = std::max(1, (int)(fl.mem_gb * 1e9 / per_point));
   const int chunk = std::min(fl.max_batch, by_mem);
 
@@ -236,17 +258,8 @@ int main(int argc, char *argv[]) {
     std::vector<opo_params> params(c1 - c0);
     for (size_t k = c0; k < c1; ++k) {
       Prepared &P = prep[k - c0];
-      const auto &v = pts[k].argv;
-      opo_args &a = P.a;
-      a.three_eqs = three; a.regime = regime;
-      a.crystal = pts[k].crystal == "PPLN" ? OPO_PPLN : OPO_SPPLT;
-      a.N = fl.N; a.NZ = fl.NZ; a.NRT = fl.NRT;
-      a.lp_nm = std::atof(v[0].c_str()); a.temp_arg = std::atof(v[1].c_str()); a.Lambda = std::atof(v[2].c_str());
-      a.Nth = std::atof(v[3].c_str()); a.R_pct = std::atof(v[4].c_str()); a.delta = std::atof(v[5].c_str());
-      a.gdd_pct = std::atof(v[6].c_str()); a.tod_pct = std::atof(v[7].c_str()); a.upm = std::atoi(v[8].c_str());
-      a.md = std::atof(v[9].c_str()); a.fm = std::atof(v[10].c_str()); a.lcr_mm = std::atof(v[11].c_str());
-      a.ls_um = fl.ls_um; a.pm_mode = fl.pm_mode;
-      CHECK(opo_derive(&a, &P.d));
+      P.a = args_of(k);
+      CHECK(opo_derive(&P.a, &P.d));
       const opo_derived &d = P.d;
       const size_t N = (size_t)d.N;
       P.T.resize(N); P.w.resize(N); P.Tp.resize(d.SIZEL); P.Fp.resize(d.SIZEL);

@@ -989,8 +989,11 @@ int opo_power_history(opo_solver *s, double *out, int n) {
   HIPCHK(hipSetDevice(s->E.device));
   Engine &E = s->E;
   Group &g = E.groups[0];
-  if (n > E.last_nrt) n = E.last_nrt;
-  HIPCHK(hipMemcpy(out, g.d_hist + E.last_rt0, n * sizeof(double), hipMemcpyDeviceToHost));
+  if (n < 0) return fail(OPO_ERR_INVALID, "n must be >= 0");
+  // the last run holds last_nrt values; the rest of out is zero-filled
+  const int have = std::min(n, E.last_nrt);
+  if (have > 0) HIPCHK(hipMemcpy(out, g.d_hist + E.last_rt0, have * sizeof(double), hipMemcpyDeviceToHost));
+  for (int k = std::max(have, 0); k < n; ++k) out[k] = 0.0;
   return OPO_OK;
 }
 
@@ -1176,11 +1179,13 @@ int opo_batch_run_ex(opo_batch *b, int max_rt, int check_every, double tol, doub
   int rc = OPO_OK;
   for (auto &g : E.groups) {
     const int n = (int)g.members.size();
-    std::vector<int> conv(n), stat(n);
+    std::vector<int> conv(n), stat(n), ran(n);
     HIPCHK(hipMemcpy(conv.data(), g.d_conv, n * sizeof(int), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(stat.data(), g.d_status, n * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ran.data(), g.d_rt, n * sizeof(int), hipMemcpyDeviceToHost));
     for (int k = 0; k < n; ++k) {
-      if (rts_out) rts_out[g.members[k]] = stat[k] < 0 ? -2 : conv[k];
+      // a diverged instance stopped at the round trip whose power was non-finite
+      if (rts_out) rts_out[g.members[k]] = stat[k] < 0 ? ran[k] : conv[k];
       if (stat[k] < 0) rc = OPO_ERR_DIVERGED;
       if (states_out) states_out[g.members[k]] = stat[k] < 0 ? -1 : (conv[k] < 0 ? 0 : (stat[k] == 1 ? 2 : 1));
     }

@@ -85,19 +85,24 @@ def run_warm(points, warm, seed=1000, sigma=1e-20, **kw):
     """run_batched from warm-start fields, with the cold start as the
     fallback: every point whose warm run diverged (non-finite power) runs
     again from the noise the cold start gives it (seed + its index), in a
-    second batch.  Its round trips add up, its state and power are the
-    fallback's, and run_s covers both runs.  Returns run_batched's dict plus
-    "fallback" (the indices that fell back) and "fallback_batch" (the second
-    batch, or None; the fallback points' fields live there, in that order)."""
+    second batch.  Its round trips add up (the warm run's up to the
+    divergence, plus the fallback's, a capped fallback counting max_rt), its
+    state and power are the fallback's, and run_s covers both runs.  Returns
+    run_batched's dict plus "fallback" (the indices that fell back) and
+    "fallback_batch" (the second batch, or None; the fallback points' fields
+    live there, in that order)."""
     r = run_batched(points, seed=seed, sigma=sigma, warm=warm, **kw)
     bad = np.flatnonzero(r["states"] == -1)
     r["fallback"], r["fallback_batch"] = bad, None
     if bad.size:
+        max_rt = kw.get("max_rt", 5000)
         cold = [(noise(seed + int(k), 0, sigma, points[k][0]), noise(seed + int(k), 1, sigma, points[k][0]))
                 for k in bad]
         f = run_batched([points[k] for k in bad], seed=seed, sigma=sigma, warm=cold, **kw)
         r["rts"] = r["rts"].copy()
-        r["rts"][bad] = np.where(r["rts"][bad] > 0, r["rts"][bad], 0) + f["rts"]
+        # a diverged instance reports the round trip it stopped at; a capped
+        # fallback reports -1 and ran max_rt
+        r["rts"][bad] = r["rts"][bad] + np.where(f["rts"] > 0, f["rts"], max_rt)
         r["states"] = r["states"].copy()
         r["states"][bad] = f["states"]
         r["power"] = r["power"].copy()

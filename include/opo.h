@@ -154,7 +154,8 @@ int opo_enable_save(opo_solver *s, int enable);
 int opo_download(opo_solver *s, double *Ap, double *As, double *Ai);
 /* slots*N complex each (slots = Nrts for CW, NRT for NS); any may be NULL */
 int opo_download_saved(opo_solver *s, double *save_p, double *save_s, double *save_i);
-/* sum |As|^2 after each of the last run's round trips (n values) */
+/* sum |As|^2 after each of the last run's round trips: n >= 0 values, those
+ * past the last run's round-trip count are 0 */
 int opo_power_history(opo_solver *s, double *out, int n);
 /* Selects kernel tiles: col_threads (per field) / row_threads in
  * {32,64,128,256,512}; 0 = default (128-byte column tiles, 64-thread row tiles).
@@ -216,7 +217,9 @@ int opo_batch_run(opo_batch *b, int max_rt, int check_every, double tol, int *rt
  * As opo_batch_run, and an instance whose power has fallen below
  * off_ratio x its power at the first check (and is still falling) is stopped
  * as "off" (below threshold).  states_out[k]: 1 steady state, 2 off, 0 not
- * converged, -1 diverged; rts_out[k] the round trips it ran (-1 / -2 as above). */
+ * converged, -1 diverged; rts_out[k] the round trips it ran: its convergence
+ * point, -1 if it did not converge, or for a diverged instance the round trip
+ * whose power was non-finite. */
 int opo_batch_run_ex(opo_batch *b, int max_rt, int check_every, double tol, double off_ratio, int *rts_out,
                      int *states_out);
 int opo_batch_download(opo_batch *b, int idx, double *Ap, double *As, double *Ai);

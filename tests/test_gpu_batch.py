@@ -106,7 +106,7 @@ def test_reference_instability_is_reproduced_and_detected(gpu):
     """Two equations on a coarse crystal grid (8 slices, 31x threshold) make the
     reference scheme itself blow up; the oracle and the HIP solver go
     non-finite at the same round trip, and the batched run reports the
-    instance as diverged (opo_batch_run -> OPO_ERR_DIVERGED, rts = -2)."""
+    instance as diverged (opo_batch_run -> OPO_ERR_DIVERGED)."""
     P = gpu
     cfg = c1(N=1024, NZ=8, NRT=1100)
     p, T, w, Ap_in, Ap, As, Ai = inputs(cfg, seed=5, sigma=1e-20)

@@ -1,6 +1,7 @@
 """The batched sweep driver (cuopo_sweep) against the sweep script
 src/cuOPO.sh: the loop nest and folder names (CPU, --dry-run) and, on the
-GPU, each point's files against a single cuOPO run of the same point."""
+GPU, each point's files against a single cuOPO run of the same point and
+against the FP64 oracle (CW and NS, two and three equations)."""
 import os
 import subprocess
 
@@ -157,3 +158,78 @@ LCR = 5
                 assert rel_l2(a, b) < 1e-13, (folder, f, kind)
         for name in ("Tp.dat", "freq.dat", "T.dat"):
             assert np.array_equal(np.loadtxt(d / name), np.loadtxt(one / name))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("reg,eqs", [("cw", "3"), ("cw", "2"), ("ns", "2"), ("ns", "3")])
+def test_batched_sweep_matches_oracle(gpu, tmp_path, reg, eqs):
+    """Every point's output files of one batched cuopo_sweep run against the
+    FP64 oracle run on the inputs the driver wrote (src/cuOPO.sh:148-164,
+    src/headers/files.cuh:21-75): the saved signal, pump (and idler) slots,
+    64 round trips for CW and all NRT round trips (SIZEL = N x NRT values)
+    for the nanosecond regime with the EOM and GDD compensation on, and the
+    T, Tp and freq grids."""
+    from oracle import oracle as O
+    N, NZ = 1024, 4
+    ns = reg == "ns"
+    NRT = 8 if ns else 70
+    spec = tmp_path / "spec.txt"
+    spec.write_text(f"""CRYSTAL = PPLN
+REG = {reg}
+EQS = {eqs}
+LP = 532
+TCR = 0
+GP = 6.97
+NN = 3 5
+RE = 97
+D = 0.05
+GD = {100 if ns else 0}
+TD = 0
+UPM = {1 if ns else 0}
+MD = 0.25
+FM = 4
+LCR = 5
+""")
+    out = tmp_path / "sweep"
+    r = run([exe("cuopo_sweep"), str(spec), f"--out={out}", f"--N={N}", f"--nz={NZ}", f"--nrt={NRT}", "--seed=70",
+             "--noise=1e-6"], tmp_path)
+    assert r.returncode == 0, r.stderr[-2000:]
+    dry = run([exe("cuopo_sweep"), str(spec), "--dry-run", "--seed=70"], tmp_path).stdout.strip().splitlines()[1:]
+    assert len(dry) == 2
+    three = eqs == "3"
+    for line in dry:
+        k, folder, args, seed = line.split("\t")
+        v = [float(x) for x in args.split()[1:]]
+        cfg = O.Config(three_eqs=three, regime=O.NS if ns else O.CW, crystal=O.PPLN, N=N, NZ=NZ, NRT=NRT,
+                       lp_nm=v[0], temp_arg=v[1], Lambda=v[2], Nth=v[3], R_pct=v[4], delta=v[5], gdd_pct=v[6],
+                       tod_pct=v[7], upm=int(v[8]), md=v[9], fm=v[10], lcr_mm=v[11])
+        p = O.setup(cfg)
+        T, w = O.grids(p)
+        Ap_in = O.pump_input(p)
+        d = out / folder
+        s0 = int(seed.split("=")[1])
+        As = read_c(str(d / "signal_input"))
+        assert rel_l2(As, O.noise(s0, 0, 1e-6, N)) < 1e-15
+        Ai = read_c(str(d / "idler_input")) if three else None
+        assert three or not (d / "idler_input_r.dat").exists()
+        sv, _ = O.run(p, w, T, Ap_in, Ap_in[:N].copy(), As, Ai, 0, NRT, save=True)
+        slots = NRT if ns else 64
+        stems = (("signal_output", "s"), ("pump_output", "p")) + ((("idler_output", "i"),) if three else ())
+        for stem, key in stems:
+            got = read_c(str(d / stem))
+            assert got.size == slots * N == (p.SIZEL if ns else 64 * N)
+            assert rel_l2(got, sv[key].reshape(-1)) < 1e-9, (folder, stem)
+        Tpo, Fpo = O.grids_long(p)
+        assert np.allclose(np.loadtxt(d / "Tp.dat"), Tpo, rtol=1e-15)
+        assert np.allclose(np.loadtxt(d / "freq.dat"), Fpo, rtol=1e-15)
+        assert np.allclose(np.loadtxt(d / "T.dat"), T, rtol=1e-15)
+
+
+def test_ns_two_equations_is_accepted(tmp_path):
+    """The nanosecond regime with two equations is a build of the reference's
+    switches (src/cuOPO.sh:34-45), so the batched driver takes it too."""
+    spec = tmp_path / "spec.txt"
+    spec.write_text(SCRIPT_SPEC.replace("REG = cw", "REG = ns"))
+    r = run([exe("cuopo_sweep"), str(spec), "--dry-run", "--seed=1"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "2 equations, ns" in r.stdout.splitlines()[0]
