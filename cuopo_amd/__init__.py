@@ -103,6 +103,7 @@ SIGNATURES = {
     "opo_nl_half": (_i, [_p, _i, _i]),
     "opo_boundary": (_i, [_p]),
     "opo_run": (_i, [_p, _i, _i]),
+    "opo_run_noalloc": (_i, [C.POINTER(Params), _p, _p, _p, C.c_size_t, _p, _i, _i, _p, _p]),
     "opo_enable_save": (_i, [_p, _i]),
     "opo_download": (_i, [_p, _p, _p, _p]),
     "opo_download_saved": (_i, [_p, _p, _p, _p]),

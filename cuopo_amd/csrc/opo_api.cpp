@@ -13,6 +13,7 @@
 #include <climits>
 #include <cmath>
 #include <complex>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -101,6 +102,12 @@ struct Group {
   double tuned_us = 0;                    // slice time of the autotuner's choice (0 = not tuned)
   bool any_gdd = false;
   std::vector<void *> allocs;
+  // no-allocation mode: device memory comes from the caller's buffers (fields,
+  // pump) and a bump pool over the static scratch array instead of hipMalloc
+  char *pool = nullptr;
+  size_t pool_cap = 0, pool_used = 0;
+  void *ext_state = nullptr;
+  const void *ext_pump = nullptr;
   hipStream_t gs = nullptr;      // the group's own stream (batches with several grid sizes)
   hipEvent_t done_ev = nullptr;  // join point of the group's chain
   hipStream_t cur = nullptr;     // stream the stage emitters launch on
@@ -127,6 +134,7 @@ struct Engine {
   bool solver = false;  // one instance that stays active: stage kernels take its pointers directly
   double off_ratio = 1e-6;
   bool inst_dirty = true;
+  bool own_stream = true;  // false: `stream` is the caller's (no-allocation entry point)
   std::vector<std::pair<int, int>> where;  // batch index -> (group, slot)
 
   hipEvent_t fork_ev = nullptr;
@@ -147,7 +155,7 @@ struct Engine {
     if (fork_ev) (void)hipEventDestroy(fork_ev);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
-    if (stream) (void)hipStreamDestroy(stream);
+    if (stream && own_stream) (void)hipStreamDestroy(stream);
     if (switch_dev) (void)hipSetDevice(prev);
   }
 };
@@ -170,6 +178,15 @@ bool group_factored(const Group &g) {
 
 int dalloc(Group &g, void **p, size_t bytes) {
   if (bytes == 0) { *p = nullptr; return OPO_OK; }
+  if (g.pool) {
+    const size_t at = (g.pool_used + 255) & ~size_t(255);
+    if (at > g.pool_cap || bytes > g.pool_cap - at)
+      return fail(OPO_ERR_INVALID, "the instance needs more device memory than the no-allocation scratch holds (" +
+                                       std::to_string(g.pool_cap >> 20) + " MiB per GPU)");
+    *p = g.pool + at;
+    g.pool_used = at + bytes;
+    return OPO_OK;
+  }
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) return fail(OPO_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
   g.allocs.push_back(*p);
@@ -295,11 +312,17 @@ int make_group(Engine &E, Group &g) {
   RCHK(dalloc(g, (void **)&g.d_partials, (size_t)n * g.N2 * sizeof(double)));
   g.hist_len = std::max(E.NRT, 1);
   RCHK(dalloc(g, (void **)&g.d_hist, (size_t)n * g.hist_len * sizeof(double)));
-  HIPCHK(hipMemsetAsync(g.d_hist, 0, (size_t)n * g.hist_len * sizeof(double), E.stream));
+  if (g.pool) {
+    const int e = opo::launch_zero(g.d_hist, (long long)n * g.hist_len, E.stream);
+    if (e != 0) return fail(OPO_ERR_HIP, std::string("zero fill: ") + hipGetErrorString((hipError_t)e));
+  } else {
+    HIPCHK(hipMemsetAsync(g.d_hist, 0, (size_t)n * g.hist_len * sizeof(double), E.stream));
+  }
   g.inst.resize(n);
   for (int k = 0; k < n; ++k) {
     Instance &I = g.inst[k];
-    RCHK(dalloc(g, &I.state, (size_t)g.NF * g.N * es));
+    if (g.ext_state) I.state = g.ext_state;  // the caller's field buffer (one instance)
+    else RCHK(dalloc(g, &I.state, (size_t)g.NF * g.N * es));
     RCHK(dalloc(g, (void **)&I.nl, (size_t)E.NZ * 16 * sizeof(double)));
   }
   pick_tiles(E, g);
@@ -326,7 +349,7 @@ int upload_instance(Engine &E, Group &g, int k, const double *w, const double *T
   const opo_params &p = I.prm;
   const int N = g.N, N1 = g.N1, NF = g.NF;
   const size_t es = esize(g);
-  if (!w || !Ap_in) return fail(OPO_ERR_INVALID, "w and Ap_in are required");
+  if (!w || (!Ap_in && !g.ext_pump)) return fail(OPO_ERR_INVALID, "w and Ap_in are required");
   const size_t need_ap = p.regime == OPO_CW ? (size_t)N : (size_t)N * p.NRT;
   if (n_ap < need_ap) return fail(OPO_ERR_INVALID, "Ap_in too short (N for CW, N*NRT for NS)");
   // position in the four-step frequency layout of natural frequency index q
@@ -336,7 +359,7 @@ int upload_instance(Engine &E, Group &g, int k, const double *w, const double *T
   // The full N-point table per field always; on the reference's grid also the
   // factorised tables (opo_phase.cpp).  group_factored picks one per group.
   {
-    I.use_fac = true;
+    I.use_fac = !g.pool;  // no-allocation mode reads the full table only
     for (int k = 0; k < 2 && I.use_fac; ++k) {
       const int spt = k ? 4 : 8;
       std::vector<double> fac((size_t)2 * NF * opo::lin_factor_len(N, N1, g.xl_bits[k], spt));
@@ -421,6 +444,13 @@ int upload_instance(Engine &E, Group &g, int k, const double *w, const double *T
   HIPCHK(hipMemcpyAsync(I.nl, nl.data(), nl.size() * sizeof(double), hipMemcpyHostToDevice, E.stream));
   HIPCHK(hipStreamSynchronize(E.stream));
   // pump (Ap_in): N (CW) or N*NRT (NS) complex
+  if (g.ext_pump) {
+    I.pump = const_cast<void *>(g.ext_pump);  // the caller's device copy, read only
+    I.pump_len = (long long)need_ap;
+    I.uploaded = true;
+    E.inst_dirty = true;
+    return OPO_OK;  // the fields are already in the caller's buffer
+  }
   if (!I.pump || I.pump_len != (long long)need_ap) {
     RCHK(dalloc(g, &I.pump, need_ap * es));
     I.pump_len = (long long)need_ap;
@@ -662,7 +692,7 @@ int engine_run(Engine &E, int rt0, int n_rt, int check_every, double tol) {
   RCHK(reset_counters(E, rt0));
   const bool graph = E.use_graph && n_rt > 2;
   if (graph) RCHK(build_graph(E, check_every, tol));
-  HIPCHK(hipEventRecord(E.ev0, E.stream));
+  if (E.ev0) HIPCHK(hipEventRecord(E.ev0, E.stream));
   RCHK(fork_groups(E));
   for (auto &g : E.groups) RCHK(emit_start(E, g));
   RCHK(join_groups(E));
@@ -687,9 +717,13 @@ int engine_run(Engine &E, int rt0, int n_rt, int check_every, double tol) {
     for (auto &g : E.groups) RCHK(emit_rt_body(E, g, true, check_every, tol));
     RCHK(join_groups(E));
   }
-  HIPCHK(hipEventRecord(E.ev1, E.stream));
-  HIPCHK(hipEventSynchronize(E.ev1));
-  HIPCHK(hipEventElapsedTime(&E.last_ms, E.ev0, E.ev1));
+  if (E.ev1) {
+    HIPCHK(hipEventRecord(E.ev1, E.stream));
+    HIPCHK(hipEventSynchronize(E.ev1));
+    HIPCHK(hipEventElapsedTime(&E.last_ms, E.ev0, E.ev1));
+  } else {
+    HIPCHK(hipStreamSynchronize(E.stream));
+  }
   E.last_rt0 = rt0; E.last_nrt = n_rt;
   // surface a launch-time fault immediately
   HIPCHK(hipGetLastError());
@@ -963,6 +997,60 @@ int opo_run(opo_solver *s, int rt0, int n_rt) {
   if (n_rt > INT_MAX - rt0) return fail(OPO_ERR_INVALID, "rt0 + n_rt overflows int");
   HIPCHK(hipSetDevice(s->E.device));
   return engine_run(s->E, rt0, n_rt, 0, 0.0);
+}
+
+// opo_run on caller-owned device buffers, with every other device buffer
+// carved from the static per-GPU scratch array: no hipMalloc, hipFree, stream,
+// event or graph is created.  Same stage kernels and default launch settings
+// as a fresh solver of the same parameters, launched eagerly on `stream`.
+int opo_run_noalloc(const opo_params *params, const double *w, const double *T, const void *d_pump, size_t n_ap,
+                    void *d_fields, int rt0, int n_rt, double *power_out, void *stream) {
+  RCHK(validate(params));
+  if (!w || !d_pump || !d_fields) return fail(OPO_ERR_INVALID, "w, d_pump and d_fields are required");
+  if (rt0 < 0 || n_rt < 1) return fail(OPO_ERR_INVALID, "rt0 must be >= 0 and n_rt >= 1");
+  if (n_rt > INT_MAX - rt0) return fail(OPO_ERR_INVALID, "rt0 + n_rt overflows int");
+  const size_t align = params->precision == OPO_FP32C ? 8 : 16;
+  if ((reinterpret_cast<uintptr_t>(d_fields) | reinterpret_cast<uintptr_t>(d_pump)) % align != 0)
+    return fail(OPO_ERR_INVALID, "d_fields and d_pump must be aligned to the complex element size");
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (params->device < 0 || params->device >= ndev) return fail(OPO_ERR_INVALID, "device ordinal out of range");
+  HIPCHK(hipSetDevice(params->device));
+  void *base = nullptr;
+  size_t cap = 0;
+  if (const int e = opo::scratch_region(&base, &cap))
+    return fail(OPO_ERR_HIP, std::string("scratch address: ") + hipGetErrorString((hipError_t)e));
+
+  Engine E;
+  E.own_stream = false;
+  E.stream = reinterpret_cast<hipStream_t>(stream);
+  E.use_graph = false;
+  E.solver = true;
+  E.count = 1; E.NZ = params->NZ; E.three_eqs = params->three_eqs ? 1 : 0; E.regime = params->regime;
+  E.fp32 = params->precision == OPO_FP32C; E.device = params->device;
+  E.NRT = std::max(params->NRT, rt0 + n_rt);  // the power history covers every round trip run
+  E.where.assign(1, {0, 0});
+  E.groups.emplace_back();
+  Group &g = E.groups[0];
+  g.N = params->N;
+  g.members.push_back(0);
+  g.any_gdd = params->GDD != 0;
+  g.cur = E.stream;
+  g.pool = static_cast<char *>(base);
+  g.pool_cap = cap;
+  g.ext_state = d_fields;
+  g.ext_pump = d_pump;
+  g.phase_mode = 0;  // the full phase table (the factorised tables are not built here)
+  RCHK(make_group(E, g));
+  g.inst[0].prm = *params;
+  RCHK(upload_instance(E, g, 0, w, T, nullptr, n_ap, nullptr, nullptr, nullptr));
+  RCHK(engine_run(E, rt0, n_rt, 0, 0.0));
+  if (power_out) {
+    HIPCHK(hipMemcpyAsync(power_out, g.d_hist + rt0, (size_t)n_rt * sizeof(double), hipMemcpyDeviceToHost,
+                          E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+  }
+  return OPO_OK;
 }
 
 int opo_download(opo_solver *s, double *Ap, double *As, double *Ai) {

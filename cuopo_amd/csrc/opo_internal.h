@@ -124,5 +124,13 @@ size_t lin_factor_len(int N, int N1, int xl_bits, int E);
 int lin_factors(const opo_params &p, const double *w, int N1, int xl_bits, int E, double *out);
 
 int launch_tick(OpoInst *insts, const TickArgs &a, void *stream);
+// zero fill of `words` 8-byte words on `stream`
+int launch_zero(void *dst, long long words, void *stream);
+
+// The no-allocation entry point's device memory: a static array of
+// kScratchBytes per GPU (opo_kernels.hip).  scratch_region returns the
+// current device's copy (address looked up once per device).
+constexpr size_t kScratchBytes = size_t(8) << 20;
+int scratch_region(void **ptr, size_t *bytes);
 
 }  // namespace opo

@@ -656,4 +656,46 @@ int launch_tick(OpoInst *insts, const TickArgs &a, void *stream) {
   return (int)hipGetLastError();
 }
 
+// zero fill of n 8-byte words (grid-stride; a kernel rather than a memset so
+// the clear is an ordinary launch on the caller's stream)
+__global__ __launch_bounds__(256) void zero_kernel(unsigned long long *__restrict__ dst, long long n) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = 0ull;
+}
+
+int launch_zero(void *dst, long long words, void *stream) {
+  if (words <= 0) return 0;
+  const long long blocks = std::min<long long>((words + 255) / 256, 1024);
+  hipLaunchKernelGGL(zero_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     (unsigned long long *)dst, words);
+  return (int)hipGetLastError();
+}
+
+// ------------------------------------------------ no-allocation scratch
+// Device memory of the no-allocation entry point: one array of kScratchBytes
+// per GPU, loaded with the code object and held until the process ends.  In
+// host code the array's name is not a device address; the address of this
+// device's copy comes from hipGetSymbolAddress, looked up once per device
+// (with that device current) and kept.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+int scratch_region(void **ptr, size_t *bytes) {
+  static std::mutex mu;
+  static void *addr[kMaxDevices] = {};
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return (int)e;
+  if (dev < 0 || dev >= kMaxDevices) return (int)hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    addr[dev] = p;
+  }
+  *ptr = addr[dev];
+  *bytes = kScratchBytes;
+  return 0;
+}
+
 }  // namespace opo

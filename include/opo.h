@@ -157,6 +157,28 @@ int opo_download_saved(opo_solver *s, double *save_p, double *save_s, double *sa
 /* sum |As|^2 after each of the last run's round trips: n >= 0 values, those
  * past the last run's round-trip count are 0 */
 int opo_power_history(opo_solver *s, double *out, int n);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): opo_run of one instance on caller-owned device buffers,
+ * calling no hipMalloc/hipFree and creating no stream, event or graph.
+ *   params        as for opo_create (validated the same way)
+ *   w, T          host grids, N doubles each (T may be NULL without the EOM)
+ *   d_pump        device pump, n_ap complex (N for CW, N*NRT for NS); read only
+ *   d_fields      device fields [Ap | As (| Ai)], N complex each, time order,
+ *                 updated in place (complex128 for FP64, complex64 for FP32C)
+ *   rt0, n_rt     round trips rt0 .. rt0+n_rt-1 (n_rt >= 1)
+ *   power_out     host, n_rt doubles (sum |As|^2 after each round trip), or NULL
+ *   stream        hipStream_t to launch on (NULL: the null stream); the call
+ *                 returns after the stream has finished the work
+ * Every other device buffer (phase, GDD and EOM tables, twiddles, slice
+ * coefficients, counters, power history) lives in one static 8 MiB array per
+ * GPU, held until the process ends.  Requests that do not fit return
+ * OPO_ERR_INVALID; FP64 three equations with the EOM and GDD tables fit up to
+ * N = 2^16 (FP32C 2^17).  No convergence test and no save slots.  Calls that
+ * overlap on one GPU (two host threads, or work still queued on another
+ * stream) share that array and race: serialise them. */
+int opo_run_noalloc(const opo_params *params, const double *w, const double *T, const void *d_pump, size_t n_ap,
+                    void *d_fields, int rt0, int n_rt, double *power_out, void *stream);
 /* Selects kernel tiles: col_threads (per field) / row_threads in
  * {32,64,128,256,512}; 0 = default (128-byte column tiles, 64-thread row tiles).
  * No reference counterpart (autotuner hook). */
