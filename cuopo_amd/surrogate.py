@@ -4,23 +4,37 @@ A small MLP (5 inputs -> 4 x 256 GELU -> 4096 outputs; BF16 weights and
 activations, FP32 accumulation, trained and evaluated by the library's own
 MFMA kernels through opo_mlp_*) maps a sweep point's (pump power over
 threshold, cavity detuning, crystal length, GDD, mirror reflectivity) to its
-steady state in a gauge-free form (the signal and idler mode phases are free;
-only their sum is fixed by the pump):
-  - the amplitude shape: |spectrum| of signal and idler on 1024 bins
-    (free-spectral-range spacing, centred on the carrier), each normalised
-    to unit norm -- the real halves of the 2 x 1024 real/imag output;
-  - the power: log10(P / Ap0^2) / 10 of each field, replicated over that
-    field's imaginary half so the loss weights it like the shape.
-An off point (decayed below threshold) has a zero shape and the power floor.
-Warm fields take the predicted shape at the predicted power with random mode
-phases.  The solver starts from that instead of noise and iterates to its own
-convergence test; the cold start stays the fallback and the yardstick of the
-prediction's error.
+steady intracavity signal and idler spectrum on a coarse grid: 1024 bins per
+field, one per free spectral range, centred on the carrier (modes -512..511).
+The 2 x 1024 complex outputs hold, per field, the gauge-free part of the
+state: the real half the amplitude spectrum normalised to unit norm, the
+imaginary half log10(P / Ap0^2) / 10 of that field (replicated over the bins so
+the loss weights the power like the shape).  Signal and idler phases are free
+up to their sum, which the pump fixes, so they are not a target.  The bins are
+counted from the cavity-resonance mode pair (m0, -m0): the pair whose idler
+walk-off phase per pass, (1/v_i - 1/v_s) L_cr w_m, cancels the mirrors'
+detuning phase 2 delta (mod 2 pi, smallest |m|).  The lasing mode is a steep
+sawtooth of detuning and crystal length (up to ~600 modes per radian), which
+the network cannot resolve from its five inputs on the carrier's grid; in the
+resonance frame its target is the small residual mode offset and the power.
 
-Data flow: `generate_dataset` runs batched sweeps (opo_batch_*) to steady
-state and encodes the fields of the points that finished; `Surrogate.fit`
-trains; `Surrogate.warm_fields` decodes predictions into time-domain fields on
-any grid N >= 1024.
+The family is the one the three-equation sPPLT workload generates (CW,
+532 nm pump, 1000 nm signal, 1137 nm idler phase-matched by the three-wave
+scan of the Lambda = 7.97 um grating; src/cuOPO.sh:73-75 gives the grating):
+pump 1.2-5 x threshold, detuning -pi..pi rad, crystal 5-40 mm, GDD -500..500
+fs^2, reflectivity 0.90-0.99.  Its steady states carry almost all their power
+in one signal/idler mode pair (m, -m), and a single pair at the right mode
+converges in a few hundred round trips where the cold start from noise needs
+thousands.  The warm start is therefore the predicted spectrum's dominant
+mode pair at the predicted power (random gauge phase); the solver iterates it
+to its own convergence test, and the cold start stays the fallback and the
+judge of the warm state's power and field.
+
+Data flow: `generate_dataset` runs batched sweeps (opo_batch_*) from seeded
+noise to the convergence test and keeps the points that reached a steady
+state (compact: top modes per field); `targets` encodes them;
+`Surrogate.fit` trains on the GPU; `Surrogate.warm_fields` decodes
+predictions into time-domain fields on any grid N >= 1024.
 """
 from __future__ import annotations
 
@@ -29,7 +43,7 @@ import math
 
 import numpy as np
 
-from . import (CW, FP32C, FP64, PPLN, Args, Batch, OpoError, _check, _ptr, derive, grids, lib,
+from . import (CW, FP32C, PM_THREE_WAVE, PPLN, SPPLT, Args, Batch, _check, _ptr, derive, grids, lib,
                noise, params_from, pump_input)
 
 NBINS = 1024
@@ -104,6 +118,53 @@ class MLP:
         return loss.value
 
 
+# ------------------------------------------------------------ the family
+
+def family_args(nth_rel, delta_rad, lcr_mm, gdd_fs2, refl, N=1 << 14, NZ=100, NRT=5000):
+    """One point of the three-equation sPPLT family (the 2^14-sample,
+    100-slice workload's generator): argv of src/cuOPO.cu:92-267 with the
+    opt-in 1000 nm signal and three-wave phase matching.  GDD [fs^2] is set
+    through the reference's GDD compensation percentage (GDD = -epsilon *
+    beta2_s * Lcr, src/cuOPO.cu:142); argv6 is detuning / pi (Q11)."""
+    a = Args(three_eqs=True, regime=CW, crystal=SPPLT, N=N, NZ=NZ, NRT=NRT, lp_nm=532.0, Lambda=7.97,
+             Nth=nth_rel, R_pct=100.0 * refl, delta=delta_rad / math.pi, lcr_mm=lcr_mm, ls_um=1.0,
+             pm_mode=PM_THREE_WAVE)
+    if gdd_fs2 != 0:
+        d = derive(a)
+        a.gdd_pct = -100.0 * (gdd_fs2 * 1e-6) / (d.b2s * d.Lcr)
+    return a
+
+
+def sweep_args(nth_rel, delta_rad, lcr_mm, gdd_fs2, refl, N=4096, NZ=25, NRT=3000, crystal=PPLN):
+    """A continuous-wave three-equation point of the degenerate PPLN sweep
+    family (532 nm pump, signal and idler at 1064 nm phase-matched by the
+    reference scan, Lambda = 6.97 um).  GDD [fs^2] as in family_args."""
+    a = Args(three_eqs=True, regime=CW, crystal=crystal, N=N, NZ=NZ, NRT=NRT, lp_nm=532.0, Lambda=6.97,
+             Nth=nth_rel, R_pct=100.0 * refl, delta=delta_rad / math.pi, lcr_mm=lcr_mm)
+    if gdd_fs2 != 0:
+        d = derive(a)
+        a.gdd_pct = -100.0 * (gdd_fs2 * 1e-6) / (d.b2s * d.Lcr)
+    return a
+
+
+def resonance_mode(d, delta_rad):
+    """The mode m0 (in free spectral ranges, on the grid of d) of the signal
+    whose pair (m0, -m0) is resonant: the idler's walk-off phase per pass
+    (1/v_i - 1/v_s) L_cr w_m0 (w stretched by N/(N-1), Q7) cancels the two
+    mirror phases 2 delta modulo 2 pi, on the branch of smallest |m0|."""
+    w1 = 2.0 * math.pi / d.t_rt * d.N / (d.N - 1)
+    c = (1.0 / d.vi - 1.0 / d.vs) * d.Lcr * w1
+    phase = (-2.0 * delta_rad + math.pi) % (2.0 * math.pi) - math.pi
+    return int(round(phase / c)) if c != 0 else 0
+
+
+def sample_points(n, seed=0):
+    """n rows of (nth_rel, delta_rad, lcr_mm, gdd_fs2, R), uniform in RANGES."""
+    rng = np.random.default_rng(seed)
+    lo, hi = RANGES[:, 0], RANGES[:, 1]
+    return lo + (hi - lo) * rng.random((n, 5))
+
+
 # ------------------------------------------------------------ encodings
 
 def encode_features(nth_rel, delta_rad, lcr_mm, gdd_fs2, refl):
@@ -112,7 +173,8 @@ def encode_features(nth_rel, delta_rad, lcr_mm, gdd_fs2, refl):
 
 
 def coarse_spectrum(A, nb=NBINS):
-    """Centred nb-bin spectrum of a time-domain field (bin spacing = FSR)."""
+    """Centred nb-bin spectrum of a time-domain field: bin nb/2 + m holds the
+    amplitude of mode m (field component e^{2 pi i m j / N}); bin spacing = FSR."""
     N = A.size
     X = np.fft.fftshift(np.fft.fft(A)) / N
     out = np.zeros(nb, complex)
@@ -132,135 +194,141 @@ def field_from_coarse(S, N):
     return np.fft.ifft(np.fft.ifftshift(X)) * N
 
 
-def gauge_fix(S, I):
-    """Signal and idler phases are free up to their sum (the pump fixes
-    phi_s + phi_i): rotate so the strongest signal bin is real positive."""
-    k = int(np.argmax(np.abs(S)))
-    ph = np.exp(-1j * np.angle(S[k])) if np.abs(S[k]) > 0 else 1.0
-    return S * ph, I * np.conj(ph)
+LOG_FLOOR = -20.0  # log10(P / Ap0^2) floor
+TARGET_KIND = "shape+logpower/resonance-frame"
+TOPK = 8  # modes kept per field in the compact dataset
 
 
-LOG_FLOOR = -20.0  # log10(P / Ap0^2) of an off point
-TARGET_KIND = "shape+logpower"
+def top_modes(A, k=TOPK, nb=NBINS):
+    """The k strongest coarse bins of a field: (bins [k] int, amplitudes [k])
+    and the field's P = sum |X|^2 over all N modes (mean |A|^2)."""
+    S = np.abs(coarse_spectrum(A, nb))
+    order = np.argsort(S)[::-1][:k]
+    X = np.fft.fft(A) / A.size
+    return order.astype(np.int32), S[order], float(np.sum(np.abs(X) ** 2))
 
 
 def _log_power(P, scale):
     return max(LOG_FLOOR, math.log10(P / scale ** 2)) if P > 0 else LOG_FLOOR
 
 
-def encode_target(As, Ai, scale):
-    """Training target of one steady state (As, Ai time-domain fields, scale =
-    the pump amplitude Ap0): [shape_s, logP_s, shape_i, logP_i], 1024 each."""
+def encode_target(bins_s, amp_s, P_s, bins_i, amp_i, P_i, scale, m0=0, nb=NBINS):
+    """Training target of one steady state from its compact form (carrier
+    bins): [shape_s, logP_s, shape_i, logP_i], nb values each, the shapes in
+    the resonance frame (signal bins shifted by -m0, idler bins by +m0; bins
+    that leave the window are dropped)."""
     out = []
-    for A in (As, Ai):
-        a = np.abs(coarse_spectrum(A))
-        P = float(np.sum(a * a))
-        lp = _log_power(P, scale)
-        shape = a / math.sqrt(P) if lp > LOG_FLOOR else np.zeros_like(a)
-        out += [shape, np.full(a.size, lp / 10.0)]
+    for bins, amp, P, sh in ((bins_s, amp_s, P_s, -m0), (bins_i, amp_i, P_i, m0)):
+        shape = np.zeros(nb)
+        b = np.asarray(bins) + sh
+        ok = (b >= 0) & (b < nb)
+        shape[b[ok]] = np.asarray(amp)[ok]
+        n = np.linalg.norm(shape)
+        out += [shape / n if n > 0 else shape, np.full(nb, _log_power(P, scale) / 10.0)]
     return np.concatenate(out).astype(np.float32)
 
 
-def decode_target(y, scale):
-    """Amplitude spectra (signal, idler) on the coarse grid from one output."""
+def decode_target(y, scale, m0=0):
+    """(carrier bin, amplitude) of signal and idler from one output: the
+    dominant bin of each predicted shape, moved back from the resonance frame,
+    at the predicted power."""
     y = np.asarray(y, np.float64)
     nb = y.size // 4
     out = []
-    for k in range(2):
-        shape = np.maximum(y[2 * k * nb:(2 * k + 1) * nb], 0.0)
+    for k, sh in ((0, m0), (1, -m0)):
+        shape = y[2 * k * nb:(2 * k + 1) * nb]
         lp = 10.0 * float(np.mean(y[(2 * k + 1) * nb:(2 * k + 2) * nb]))
-        n = float(np.linalg.norm(shape))
-        if lp <= LOG_FLOOR + 1.0 or n == 0:
-            out.append(np.zeros(nb))
-            continue
-        out.append(shape / n * scale * 10.0 ** (lp / 2))
-    return out[0], out[1]
+        out.append((int(np.argmax(shape)) + sh, scale * 10.0 ** (lp / 2)))
+    return out
 
 
-# ------------------------------------------------------------ sweep points
+# ------------------------------------------------------------ data
 
-def sweep_args(nth_rel, delta_rad, lcr_mm, gdd_fs2, refl, N=4096, NZ=25, NRT=3000, crystal=PPLN):
-    """A continuous-wave three-equation point of the sweep family (PPLN,
-    532 nm pump, degenerate signal/idler phase-matched by the reference scan,
-    Lambda = 6.97 um).  GDD [fs^2] is set through the reference's GDD
-    compensation percentage (GDD = -epsilon * beta2_s * Lcr, cuOPO.cu:142)."""
-    a = Args(three_eqs=True, regime=CW, crystal=crystal, N=N, NZ=NZ, NRT=NRT, lp_nm=532.0, Lambda=6.97,
-             Nth=nth_rel, R_pct=100.0 * refl, delta=delta_rad / math.pi, lcr_mm=lcr_mm)
-    d = derive(a)
-    if gdd_fs2 != 0:
-        a.gdd_pct = -100.0 * (gdd_fs2 * 1e-6) / (d.b2s * d.Lcr)
-    return a
-
-
-def sample_points(n, seed=0):
-    rng = np.random.default_rng(seed)
-    lo, hi = RANGES[:, 0], RANGES[:, 1]
-    return lo + (hi - lo) * rng.random((n, 5))
-
-
-def _instance(a, precision, As, Ai):
-    d = derive(a)
-    p = params_from(d, precision)
-    T, w = grids(d)
-    Ap_in = pump_input(d)
-    return p, d, (w, T, Ap_in, Ap_in[: d.N].copy(), As, Ai)
-
-
-def run_sweep(points, N=4096, NZ=25, max_rt=3000, check_every=50, tol=1e-6, precision=FP32C, seed=0,
-              warm=None, noise_sigma=1e-20, tiles=(0, 0)):
-    """Run sweep points (rows of [nth_rel, delta, lcr_mm, gdd_fs2, R]) in one
-    batch to convergence.  warm: optional list of (As, Ai) initial fields.
-    Returns (fields, rts, final power, device ms, derived scalars, states
-    (1 steady, 2 off, 0 capped, -1 diverged))."""
+def family_batch(points, N, NZ, max_rt, precision=FP32C, seed=0, noise_sigma=1e-20, warm=None):
+    """An uploaded Batch of family points (rows of RANGES order) from seeded
+    noise (seed + k, streams 0 / 1) or from warm (As, Ai) fields; also the
+    derived scalars of each point."""
     made = []
     for k, pt in enumerate(points):
-        a = sweep_args(*pt, N=N, NZ=NZ, NRT=max_rt)
+        d = derive(family_args(*pt, N=N, NZ=NZ, NRT=max_rt))
+        T, w = grids(d)
+        Ap_in = pump_input(d)
         if warm is not None:
             As, Ai = warm[k]
         else:
-            As = noise(seed + k, 0, noise_sigma, N)
-            Ai = noise(seed + k, 1, noise_sigma, N)
-        made.append(_instance(a, precision, As, Ai))
+            As, Ai = noise(seed + k, 0, noise_sigma, N), noise(seed + k, 1, noise_sigma, N)
+        made.append((params_from(d, precision), d, (w, T, Ap_in, Ap_in[:N].copy(), As, Ai)))
     b = Batch([m[0] for m in made])
-    if tiles != (0, 0):
-        b.set_tiles(*tiles)
     for k, m in enumerate(made):
         b.upload(k, *m[2])
+    return b, [m[1] for m in made]
+
+
+def run_family(points, N=4096, NZ=25, max_rt=20000, check_every=50, tol=1e-6, off_ratio=1e-6, precision=FP32C,
+               seed=0, noise_sigma=1e-20, warm=None, autotune=True, fields=True):
+    """Run family points in one batch to the convergence test.  Returns a dict:
+    rts, states (1 steady, 2 off, 0 capped, -1 diverged), power, device ms,
+    the derived scalars and (fields=True) the final (Ap, As, Ai) per point."""
+    b, ds = family_batch(points, N, NZ, max_rt, precision, seed, noise_sigma, warm)
     try:
-        rts, states = b.run_ex(max_rt, check_every=check_every, tol=tol, off_ratio=1e-6)
-    except OpoError as e:
-        if e.status != -5:
-            raise
-        rts, states = None, None
-    fields = [b.download(k) for k in range(len(points))]
-    power = b.power()
-    ms = b.last_run_ms()
-    b.close()
-    return fields, rts, power, ms, [m[1] for m in made], states
+        if autotune:
+            b.autotune(iters=3)
+        rts, st = b.run_ex(max_rt, check_every=check_every, tol=tol, off_ratio=off_ratio)
+        out = dict(rts=rts, states=st, power=b.power(), device_ms=b.last_run_ms(), derived=ds)
+        if fields:
+            out["fields"] = [b.download(k) for k in range(len(points))]
+    finally:
+        b.close()
+    return out
 
 
-def generate_dataset(n, seed=0, chunk=256, **kw):
-    """Encoded finished points (steady or off) of n random sweep points:
-    (features [m,5], targets [m,4096], scales [m], rts [m], states [m])."""
+def generate_dataset(n, seed=0, chunk=512, N=4096, NZ=25, max_rt=20000, noise_sigma=1e-20, precision=FP32C,
+                     log=None):
+    """Steady states of n random family points, cold-started from seeded
+    noise.  Returns a dict of arrays over the points that reached the steady
+    test: X features [m, 5], points [m, 5], scale (Ap0) [m], rts [m], and per
+    field f in (s, i): bins_f [m, TOPK], amp_f [m, TOPK], P_f [m]; plus counts
+    of every finish state over all n points."""
     pts = sample_points(n, seed)
-    X, Y, S, R, ST = [], [], [], [], []
-    for s in range(0, n, chunk):
-        part = pts[s:s + chunk]
-        fields, rts, power, ms, ds, states = run_sweep(part, seed=seed * 100003 + s, **kw)
-        if states is None:
-            continue
+    keep = {k: [] for k in ("X", "points", "scale", "rts", "bins_s", "amp_s", "P_s", "bins_i", "amp_i", "P_i")}
+    counts = {"steady": 0, "off": 0, "capped": 0, "diverged": 0}
+    for s0 in range(0, n, chunk):
+        part = pts[s0:s0 + chunk]
+        r = run_family(part, N=N, NZ=NZ, max_rt=max_rt, precision=precision, seed=seed * 1000003 + s0,
+                       noise_sigma=noise_sigma)
         for k, pt in enumerate(part):
-            _, As, Ai = fields[k]
-            if states[k] not in (1, 2) or not (np.isfinite(As).all() and np.isfinite(Ai).all()):
+            st = int(r["states"][k])
+            counts[{1: "steady", 2: "off", 0: "capped", -1: "diverged"}[st]] += 1
+            _, As, Ai = r["fields"][k]
+            if st != 1 or not (np.isfinite(As).all() and np.isfinite(Ai).all()):
                 continue
-            scale = ds[k].Ap0
-            X.append(encode_features(*pt))
-            Y.append(encode_target(As, Ai, scale) if states[k] == 1 else
-                     encode_target(np.zeros_like(As), np.zeros_like(Ai), scale))
-            S.append(scale)
-            R.append(int(rts[k]))
-            ST.append(int(states[k]))
-    return np.array(X), np.array(Y), np.array(S), np.array(R), np.array(ST)
+            keep["X"].append(encode_features(*pt))
+            keep["points"].append(pt)
+            keep["scale"].append(r["derived"][k].Ap0)
+            keep["rts"].append(int(r["rts"][k]))
+            for f, A in (("s", As), ("i", Ai)):
+                b_, a_, P = top_modes(A)
+                keep["bins_" + f].append(b_)
+                keep["amp_" + f].append(a_)
+                keep["P_" + f].append(P)
+        if log:
+            log(s0 + len(part), dict(counts))
+    out = {k: np.array(v) for k, v in keep.items()}
+    out.update({k: np.array(v) for k, v in counts.items()})
+    return out
+
+
+def frames(points, N, NZ):
+    """Resonance-frame offsets m0 of family points on grid N."""
+    return [resonance_mode(derive(family_args(*p, N=N, NZ=NZ)), p[1]) for p in points]
+
+
+def targets(ds, N=4096, NZ=25):
+    """Training targets [m, 4096] of a dataset from generate_dataset (grid N
+    x NZ, the grid the dataset was made on)."""
+    m0 = frames(ds["points"], N, NZ)
+    return np.array([encode_target(ds["bins_s"][j], ds["amp_s"][j], ds["P_s"][j], ds["bins_i"][j], ds["amp_i"][j],
+                                   ds["P_i"][j], ds["scale"][j], m0[j]) for j in range(len(ds["scale"]))])
 
 
 class Surrogate:
@@ -291,30 +359,31 @@ class Surrogate:
     def predict(self, X):
         return self.net.forward(X)
 
-    def warm_fields_mixed(self, points, Ns, scales, floor_sigma=1e-12, seed=0, use_mean=False):
-        """warm_fields for points on different grids (Ns[k] for point k)."""
-        out = []
-        for k, (pt, N) in enumerate(zip(points, Ns)):
-            out.append(self.warm_fields([pt], int(N), [scales[k]], floor_sigma, seed + 7919 * k, use_mean)[0])
-        return out
-
-    def warm_fields(self, points, N, scales, floor_sigma=1e-12, seed=0, use_mean=False):
-        """Predicted (As, Ai) time-domain fields for sweep points on grid N
-        (use_mean: the mean training target for every point instead)."""
+    def modes(self, points, scales, m0, use_mean=False):
+        """Per point: ((signal bin, amplitude), (idler bin, amplitude)),
+        carrier bins; m0 the points' resonance frames."""
         if use_mean:
             if self.mean_y is None:
                 raise ValueError("no mean target: fit or load a trained surrogate first")
             Y = np.tile(self.mean_y, (len(points), 1))
         else:
             Y = self.predict(np.array([encode_features(*p) for p in points]))
+        return [decode_target(Y[k], scales[k], m0[k]) for k in range(len(points))]
+
+    def warm_fields(self, points, N, scales, seed=0, use_mean=False, NZ=100):
+        """Warm-start (As, Ai) time-domain fields on grid N: the predicted
+        dominant signal and idler modes at the predicted amplitudes, each with
+        a random phase (use_mean: the mean training target for every point,
+        i.e. the resonance pair at the mean power)."""
         rng = np.random.default_rng(seed)
         out = []
-        for k in range(len(points)):
-            aS, aI = decode_target(Y[k], scales[k])
-            # predicted amplitudes, free mode phases drawn at random
-            S = aS * np.exp(2j * np.pi * rng.random(aS.size))
-            I = aI * np.exp(2j * np.pi * rng.random(aI.size))
-            As = field_from_coarse(S, N) + noise(seed + k, 0, floor_sigma, N)
-            Ai = field_from_coarse(I, N) + noise(seed + k, 1, floor_sigma, N)
-            out.append((As, Ai))
+        m0 = frames(points, N, NZ)
+        for (bs, a_s), (bi, a_i) in self.modes(points, scales, m0, use_mean):
+            S = np.zeros(N, complex)
+            I = np.zeros(N, complex)
+            # carrier bin b = NBINS/2 + m holds mode m: on the N-point grid
+            # (fftshifted) that is index N/2 + m
+            S[(N // 2 + bs - NBINS // 2) % N] = a_s * np.exp(2j * np.pi * rng.random())
+            I[(N // 2 + bi - NBINS // 2) % N] = a_i * np.exp(2j * np.pi * rng.random())
+            out.append((np.fft.ifft(np.fft.ifftshift(S)) * N, np.fft.ifft(np.fft.ifftshift(I)) * N))
         return out

@@ -114,11 +114,19 @@ def run_warm(points, warm, seed=1000, sigma=1e-20, **kw):
 
 
 def run_single_fp64(pt, NZ=100, max_rt=5000, check_every=50, tol=1e-6, off_ratio=1e-6, seed=1000, k=0,
-                    sigma=1e-20, autotune=True):
+                    sigma=1e-20, autotune=True, args=None):
     """Plain per-instance FP64 cold start: one solver, round trips in chunks of
     check_every with the same convergence and off tests evaluated on the host
-    from the power history (the way a per-point driver would)."""
-    p, d, up = _setup(pt, NZ, max_rt, FP64, seed, k, sigma, None)
+    from the power history (the way a per-point driver would).  pt is a row of
+    sweep_points, or with `args` given (an Args of any family) ignored."""
+    if args is None:
+        p, d, up = _setup(pt, NZ, max_rt, FP64, seed, k, sigma, None)
+    else:
+        d = derive(args)
+        p = params_from(d, FP64)
+        T, w = grids(d)
+        Ap_in = pump_input(d)
+        up = (w, T, Ap_in, Ap_in[:d.N].copy(), noise(seed + k, 0, sigma, d.N), noise(seed + k, 1, sigma, d.N))
     s = Solver(p)
     s.upload(*up)
     if autotune:
@@ -144,5 +152,6 @@ def run_single_fp64(pt, NZ=100, max_rt=5000, check_every=50, tol=1e-6, off_ratio
             first = P
         prev = P
     wall = time.perf_counter() - t0
+    fields = s.download()
     s.close()
-    return dict(rts=done, state=state, power=P, wall_s=wall)
+    return dict(rts=done, state=state, power=P, wall_s=wall, fields=fields)

@@ -1,13 +1,15 @@
 """What the trained surrogate gains over the method without it (GPU).
 
 The committed network (cuopo_amd/data/surrogate.npz, trained on the GPU by
-python -m cuopo_amd.train_surrogate) is run on sweep points it never saw: the
-training generator's settings with another seed.  Without the network, the
-best input-independent guess of a point's steady state is the mean training
-target (the ablation the A/B uses).  The network's gain is in the steady
-intracavity power: its predicted log10(P / Ap0^2) must beat that mean's.  (The
-spectral shape is multistable and not predictable from the inputs, and the
-warm start saves no round trips; both are measured by the A/B script.)"""
+python -m cuopo_amd.train_surrogate) warm-starts points of its family it never
+saw (another seed), on the grid it was trained on, in FP32C.  Without it, a
+point starts from the workload's 1e-20 noise.  On the points where both runs
+reach the steady-state test, the warm start must need far fewer round trips,
+and mostly end in the cold start's state: the same signal mode and steady
+power.  The mean training target (the resonance-frame mode pair at the mean
+power, the same for every point) is the ablation: the network must pick the
+cold start's mode more often.  (The A/B at the workload's full grid in FP64
+is cuopo_amd.ab_surrogate.)"""
 import os
 
 import numpy as np
@@ -19,24 +21,37 @@ WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
                        "surrogate.npz")
 
 
-def test_network_predicts_steady_power_better_than_the_mean_target(gpu):
-    from cuopo_amd import FP32C
+def test_warm_start_saves_round_trips_and_reaches_the_cold_state(gpu):
     from cuopo_amd import surrogate as S
     from cuopo_amd.train_surrogate import load
     sur, meta = load(WEIGHTS, max_batch=64)
-    # the training generator's settings (meta), fresh points: training used seed 1
-    X, Y, scales, R, ST = S.generate_dataset(64, seed=2024, N=meta["N"], NZ=meta["NZ"], max_rt=meta["max_rt"],
-                                             check_every=50, tol=1e-6, precision=FP32C,
-                                             noise_sigma=meta["seed_noise"])
-    assert X.shape[0] >= 16, f"only {X.shape[0]} of 64 points finished"
-    nb = Y.shape[1] // 4
-    true = 10 * Y[:, nb:2 * nb].mean(axis=1)
-    pred = 10 * sur.predict(X)[:, nb:2 * nb].mean(axis=1)
-    mean = 10 * float(sur.mean_y[nb:2 * nb].mean())
-    rmse_net = float(np.sqrt(np.mean((pred - true) ** 2)))
-    rmse_mean = float(np.sqrt(np.mean((mean - true) ** 2)))
-    print(f"finished {X.shape[0]} of 64 ({int((ST == 1).sum())} steady, {int((ST == 2).sum())} off); "
-          f"signal log10 power RMSE: network {rmse_net:.3f}, mean target {rmse_mean:.3f} decades")
+    N, NZ, cap = meta["N"], meta["NZ"], 20000
+    pts = S.sample_points(48, seed=2024)  # training used seed 1
+    ap0 = [S.derive(S.family_args(*p, N=N, NZ=NZ)).Ap0 for p in pts]
+    # default launch settings (no autotuning): the same arithmetic, hence the
+    # same outcome, on every run
+    cold = S.run_family(pts, N=N, NZ=NZ, max_rt=cap, seed=1000, autotune=False)
+    runs = {name: S.run_family(pts, N=N, NZ=NZ, max_rt=cap, autotune=False,
+                               warm=sur.warm_fields(pts, N, ap0, seed=3, use_mean=name == "mean", NZ=NZ))
+            for name in ("network", "mean")}
     sur.net.close()
-    assert np.isfinite(pred).all()
-    assert rmse_net < rmse_mean
+
+    def mode(A):
+        return int(np.argmax(np.abs(np.fft.fft(A))))
+
+    stats = {}
+    for name, r in runs.items():
+        both = np.flatnonzero((r["states"] == 1) & (cold["states"] == 1))
+        rel = np.abs(r["power"][both] - cold["power"][both]) / cold["power"][both]
+        same = [mode(r["fields"][k][1]) == mode(cold["fields"][k][1]) for k in both]
+        stats[name] = dict(both=int(both.size), rts=float(r["rts"][both].sum()),
+                           cold_rts=float(cold["rts"][both].sum()), rel_power_median=float(np.median(rel)),
+                           same_mode=float(np.mean(same)))
+    print(f"cold: {int((cold['states'] == 1).sum())} steady of {len(pts)}; {stats}")
+    net = stats["network"]
+    assert net["both"] >= 16
+    assert net["rts"] < 0.2 * net["cold_rts"]  # at least 5x fewer round trips (measured ~10x)
+    # measured: the cold start's mode in ~0.7 of the points, median power 5e-5 from the cold start's
+    assert net["same_mode"] >= 0.5 and net["rel_power_median"] < 1e-3
+    # the network's gain over the ablation is the mode it picks (measured 0.69 vs 0.57)
+    assert stats["mean"]["same_mode"] < net["same_mode"]

@@ -40,6 +40,8 @@ def test_gelu_derivative():
 
 
 def test_spectrum_encoding_round_trip():
+    """Coarse spectrum <-> field, and the compact steady-state target in the
+    resonance frame decodes back to the same carrier modes and powers."""
     import cuopo_amd.surrogate as S
     rng = np.random.default_rng(1)
     N = 4096
@@ -49,24 +51,40 @@ def test_spectrum_encoding_round_trip():
     c = S.coarse_spectrum(A)
     assert c.size == 1024
     assert np.allclose(S.field_from_coarse(c, N), A, atol=1e-12)
-    # gauge fix keeps the signal-idler phase sum and makes the peak real positive
-    I = rng.standard_normal(1024) + 1j * rng.standard_normal(1024)
-    s2, i2 = S.gauge_fix(c, I)
-    k = np.argmax(np.abs(c))
-    assert abs(s2[k].imag) < 1e-12 and s2[k].real > 0
-    assert np.allclose(s2 * i2, c * I)
-    y = S.encode_target(A, 0.5 * A, 2.0)
-    s3, i3 = S.decode_target(y, 2.0)
-    assert np.allclose(s3, np.abs(c), rtol=1e-5, atol=1e-6 * np.abs(c).max())
-    assert np.allclose(i3, 0.5 * np.abs(c), rtol=1e-5, atol=1e-6 * np.abs(c).max())
-    # shape halves have unit norm, power halves are constant
+    # a single mode pair (m, -m) = (37, -37) with a weak neighbour
+    j = np.arange(N)
+    As = 0.3 * np.exp(2j * np.pi * 37 * j / N) + 0.001 * np.exp(2j * np.pi * 38 * j / N)
+    Ai = 0.25 * np.exp(-2j * np.pi * 37 * j / N + 0.4j)
+    bs, a_s, P_s = S.top_modes(As)
+    bi, a_i, P_i = S.top_modes(Ai)
+    assert bs[0] == 512 + 37 and bi[0] == 512 - 37
+    assert abs(P_s - (0.09 + 1e-6)) < 1e-12 and abs(P_i - 0.0625) < 1e-12
     nb = 1024
-    assert abs(np.linalg.norm(y[:nb]) - 1) < 1e-5 and np.ptp(y[nb:2 * nb]) == 0
-    # an off point: zero shape, power floor, decodes to zero fields
-    z = S.encode_target(np.zeros(N, complex), np.zeros(N, complex), 2.0)
-    assert np.all(z[:nb] == 0) and np.allclose(z[nb:2 * nb], S.LOG_FLOOR / 10)
-    s4, i4 = S.decode_target(z, 2.0)
-    assert not s4.any() and not i4.any()
+    for m0 in (0, 30, -5):
+        y = S.encode_target(bs, a_s, P_s, bi, a_i, P_i, scale=2.0, m0=m0)
+        # shapes have unit norm and sit at the offset from the resonance pair
+        assert abs(np.linalg.norm(y[:nb]) - 1) < 1e-6 and np.argmax(y[:nb]) == 512 + 37 - m0
+        assert np.argmax(y[2 * nb:3 * nb]) == 512 - 37 + m0
+        assert np.ptp(y[nb:2 * nb]) == 0  # replicated log power
+        (ds, das), (di, dai) = S.decode_target(y, 2.0, m0)
+        assert (ds, di) == (512 + 37, 512 - 37)
+        assert abs(das - np.sqrt(P_s)) < 1e-6 * np.sqrt(P_s) and abs(dai - 0.25) < 1e-6
+
+
+def test_resonance_mode_cancels_the_detuning_phase():
+    """The resonance pair's idler walk-off phase per pass cancels the two
+    mirror phases 2 delta (mod 2 pi) to within half a mode, on the branch of
+    smallest |m|, for points of the sPPLT family."""
+    import math
+
+    import cuopo_amd.surrogate as S
+    for pt in S.sample_points(50, seed=9):
+        d = S.derive(S.family_args(*pt, N=4096, NZ=25))
+        m0 = S.resonance_mode(d, pt[1])
+        c = (1 / d.vi - 1 / d.vs) * d.Lcr * 2 * math.pi / d.t_rt * d.N / (d.N - 1)
+        r = (2 * pt[1] + c * m0 + math.pi) % (2 * math.pi) - math.pi
+        assert abs(r) <= 0.5 * abs(c) + 1e-12
+        assert abs(c * m0) <= math.pi + 0.5 * abs(c)
 
 
 def test_feature_encoding_range():
