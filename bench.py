@@ -1,23 +1,26 @@
 """Benchmark of the SSFM round-trip solver on one MI355X.
 
-Step = one cavity round trip (pump re-injection, NZ split-step slices of the
-three coupled-wave equations through the crystal, cavity boundary) of one
-instance, FP64 complex, fields resident in HBM.  Workload: continuous-wave
-sPPLT OPO, three equations (pump 532 nm, signal 1000 nm, idler 1137 nm),
-20 mm crystal, 2^14 complex128 samples per round trip, 100 z-slices per pass,
-pump 3x threshold, detuning 0.3 rad, signal/idler seeded with 1e-20 complex
-Gaussian noise.
+Headline: sweep points per second to steady state.  A 256-point sweep of the
+continuous-wave three-equation sPPLT family (pump 532 nm, signal 1000 nm,
+idler 1137 nm; pump 1.2-5x threshold, detuning -pi..pi rad, crystal 5-40 mm,
+GDD -500..500 fs^2, reflectivity 0.90-0.99) at the workload's grid (2^14
+complex128 samples per round trip, 100 z-slices per pass) runs as one FP64
+batch from the learned warm start (the MLP's predicted mode pair and power)
+to the steady-state test: intracavity power change below 1e-6 over 50 round
+trips, cap 20,000 round trips (a cold start from 1e-20 noise needs ~10^4).  Counted: points that pass that test.  Beside it,
+the plain per-instance FP64 solver cold-starts from 1e-20 noise on a subset
+of the same points, one point after another, to the same test or cap, and
+its steady points per second are measured, as is the warm start's steady
+power against the cold start's where both reach steady state.
 
-Prints one JSON line: round trips/s, the roofline of the dominant kernel
-(per-launch kernel time measured with start/stop events on the solver's
-stream) and a CPU baseline (the FP64 oracle on one of this host's cores).
-Under "extra": the per-slice time inside the round-trip graph, the round-trip
-HBM fraction and FP64 flop rate, one plain copy launch of each shape's stage
-bytes (the same-size ceiling), the upload/download times and the rate with
-them included (the C ABI takes host buffers), the N = 2^18 single instance and the 256
-instance FP64 batch, the 256-point sweep (batched FP32C against the plain
-per-instance FP64 solver, with the steady-power check against FP64 cold
-starts), and the CPU baselines of the other workloads (one core, all cores).
+Prints one JSON line: the headline, the roofline of the batch's dominant
+kernel (per-launch kernel time measured with start/stop events on the
+batch's stream) and a CPU baseline (the FP64 oracle on this host's cores, a
+bounded sample scaled to the same unit).  Under "extra": the single-instance
+round trips/s of the 2^14-sample workload (with its kernel roofline and the
+PCIe-inclusive rate), the N = 2^18 pulsed single instance, the 256-instance
+degenerate-PPLN batch (FP64 round trips, and FP32C to steady state from
+noise), and the other CPU baselines.
 """
 import argparse
 import json
@@ -25,6 +28,8 @@ import math
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -62,9 +67,6 @@ def workload_c5(N=1 << 18, NZ=200, NRT=40):
 
 
 TUNING = None  # (col_threads, row_threads, xcd_pack, phase_mode[, col_spt, row_spt]) instead of autotuning
-
-
-SWEEP_RTS = []  # (points, round trips each needed) of the batched cold sweep, for the CPU estimate
 
 
 def log(msg):
@@ -215,10 +217,10 @@ def median_rate(inst, seconds, reps=5):
     return float(np.median([oracle_rate(inst, seconds / reps) for _ in range(reps)]))
 
 
-def cpu_baseline(args_c2, seconds=12.0):
-    """Headline CPU baseline: the FP64 oracle (a scalar C transcription of the
-    reference kernels with the project's radix-2 FFT) on one core, round
-    trips of the same workload, median of 5 timings."""
+def cpu_c2_1core(args_c2, seconds=12.0):
+    """The FP64 oracle (a scalar C transcription of the reference kernels with
+    the project's radix-2 FFT) on one core, round trips of the 2^14-sample
+    workload, median of 5 timings."""
     model, ncpu, _ = cpu_host()
     v = median_rate(oracle_instance(args_c2), seconds)
     return {"value": v, "unit": "round trips/s", "cores": 1, "kind": "port",
@@ -227,17 +229,56 @@ def cpu_baseline(args_c2, seconds=12.0):
                       f"({ncpu} cores available)"}
 
 
-def cpu_extra(seconds=8.0, sweep=None):
+def threaded_rates(insts, seconds):
+    """Round trips/s of the oracle on each of len(insts) threads at once (the
+    oracle releases the interpreter lock in its C calls)."""
+    import threading
+    rates = [0.0] * len(insts)
+    start = threading.Barrier(len(insts))
+
+    def work(k):
+        rates[k] = oracle_rate(insts[k], seconds, start)
+
+    th = [threading.Thread(target=work, args=(k,)) for k in range(len(insts))]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    return rates
+
+
+def cpu_headline(cold, seconds=6.0):
+    """CPU baseline in the headline's unit: the FP64 oracle runs the plain
+    cold-start method one point per core on all cores.  Sample: every thread
+    times round trips of one sweep point at the workload's grid for `seconds`;
+    the per-core rate then prices the round trips each subset point needed in
+    the per-instance FP64 cold start (the same algorithm), spread evenly over
+    the threads, and the subset's steady points are counted."""
+    from cuopo_amd import surrogate as S
+    model, ncpu, nthr = cpu_host()
+    pts = cold["points"]
+    insts = [oracle_instance(S.family_args(*pts[k % len(pts)], N=1 << 14, NZ=100, NRT=SWEEP_CAP))
+             for k in range(nthr)]
+    rates = threaded_rates(insts, seconds)
+    per_core = float(np.mean(rates))
+    core_s = float(sum(cold["rts"]) / per_core)
+    wall = core_s / nthr
+    steady = int(sum(1 for st in cold["states"] if st == 1))
+    return {"value": steady / wall, "unit": "sweep points/s to steady state", "cores": nthr, "kind": "port",
+            "sample": f"{nthr} threads x {seconds:.0f} s of FP64 oracle round trips at 2^14 samples x 100 slices "
+                      f"({per_core:.3f} round trips/s per core, radix-2 FFT, no FFTW) pricing the "
+                      f"{int(sum(cold['rts']))} round trips the {len(pts)}-point cold-start subset ran to "
+                      f"{steady} steady states; {model}, {ncpu} cores available",
+            "round_trips_per_s_per_core": per_core}
+
+
+def cpu_extra(seconds=8.0, c3_cold=None):
     """The other CPU baselines: C1 and C5 single instance on one core (C5 timed
     on 20 slices per pass and scaled to 200: the boundary work is counted ten
-    times, so the C5 figure is slightly low), and the sweep family with one
-    instance per core on all cores (threads over the oracle, which releases
-    the interpreter lock in its C calls): round trips/s per core per grid size,
-    and, when the sweep section ran, the sweep points/s those rates give for
-    the round trips each point needed in the batched run, with perfect load
-    balance over the cores."""
-    import threading
-    import numpy as np
+    times, so the C5 figure is slightly low), and the degenerate-PPLN sweep
+    family with one instance per core on all cores: round trips/s per core per
+    grid size and, when its cold sweep ran, the points/s (any finish state)
+    those rates give for the round trips each point ran there."""
     from cuopo_amd import sweep as SW
     from cuopo_amd.surrogate import sweep_args
     model, ncpu, nthr = cpu_host()
@@ -253,197 +294,158 @@ def cpu_extra(seconds=8.0, sweep=None):
     for N in Ns:
         _, nth, delta, lcr, gdd, R = firsts[N]
         insts.append(oracle_instance(sweep_args(nth, delta, lcr, gdd, R, N=N, NZ=100, NRT=5000)))
-    rates = [0.0] * nthr
-    start = threading.Barrier(nthr)
-
-    def work(k):
-        rates[k] = oracle_rate(insts[k], seconds, start)
-
-    th = [threading.Thread(target=work, args=(k,)) for k in range(nthr)]
-    for t in th:
-        t.start()
-    for t in th:
-        t.join()
+    rates = threaded_rates(insts, seconds)
     per_n = {N: float(np.mean([r for r, n in zip(rates, Ns) if n == N])) for N in SW.SWEEP_N if N in Ns}
     c3 = {"round_trips_per_s_per_core": {str(N): v for N, v in per_n.items()}, "threads": nthr}
-    if sweep is not None and len(per_n) == len(SW.SWEEP_N):
-        sweep_pts, rts = sweep
+    if c3_cold is not None and len(per_n) == len(SW.SWEEP_N):
+        sweep_pts, rts = c3_cold
         core_s = float(sum(rts[k] / per_n[p[0]] for k, p in enumerate(sweep_pts)))
-        c3["est_sweep_points_per_s"] = len(sweep_pts) / (core_s / nthr)
-        c3["estimate"] = ("sum over the sweep points of the round trips each ran in the batched FP32C cold run / "
-                          "the per-core oracle rate at its grid size, spread evenly over the threads")
+        c3["finished_points_per_s"] = len(sweep_pts) / (core_s / nthr)
     out["c3_sweep_all_cores"] = c3
     return out
 
 
-def sweep_section(weights):
-    """Sweep points/s on the 256-point continuous-wave three-equation sweep
-    family (cuopo_amd.sweep): batched FP32-complex from noise, batched from the
-    surrogate's warm start and from the mean-target seed (ablation), and the
-    plain per-instance FP64 solver."""
-    import numpy as np
+def c3_sweep_cold():
+    """The degenerate-PPLN 256-point family (N in 2^12..2^15, 100 slices) as
+    one FP32C batch from 1e-20 noise to the steady / off tests or 5000 round
+    trips.  Its steady states are multimode and drift slowly, so few pass the
+    test; reported as finish-state counts and round trips."""
     import cuopo_amd as P
     from cuopo_amd import sweep as SW
     pts = SW.sweep_points(256, seed=3)
-    out = {"points": len(pts), "family": "CW three-equation MgO:PPLN, 20 mm, 100 slices/pass, N in 2^12..2^15, "
-                                          "pump 1.2-5x threshold, detuning U[-pi,pi] rad, R U[0.90,0.99]; "
-                                          "finished = steady (power change <1e-6 over 50 round trips) or off "
-                                          "(decayed below threshold) or 5000-round-trip cap"}
-
-    def summarise(r):
-        st = r["states"]
-        rts = np.where(r["rts"] > 0, r["rts"], 5000)
-        return {"run_s": r["run_s"], "points_per_s": len(pts) / r["run_s"], "steady": int((st == 1).sum()),
-                "off": int((st == 2).sum()), "capped": int((st == 0).sum()), "diverged": int((st == -1).sum()),
-                "mean_round_trips": float(rts.mean())}, rts
-
-    cold = SW.run_batched(pts, precision=P.FP32C)
-    out["batched_fp32c_cold"], cold_rts = summarise(cold)
-    SWEEP_RTS[:] = [(pts, cold_rts)]
-    cold_power, cold_states, ap0 = cold["power"], cold["states"], cold["ap0"]
-    arms = {"batched_fp32c_cold": (cold_states, cold_power)}
-    cold["batch"].close()
-    # plain FP64 per-instance solver: measured round-trip time per grid size
-    per_rt = {}
-    for N in SW.SWEEP_N:
-        pt = next(p for p in pts if p[0] == N)
-        o = SW.run_single_fp64(pt, max_rt=200, tol=0.0)
-        per_rt[N] = o["wall_s"] / o["rts"]
-    est = float(sum(cold_rts[k] * per_rt[p[0]] for k, p in enumerate(pts)))
-    out["per_instance_fp64_cold"] = {"seconds_per_round_trip": {str(k): v for k, v in per_rt.items()},
-                                     "estimated_sweep_s": est, "points_per_s": len(pts) / est,
-                                     "estimate": "sum over points of the round trips each point ran in the "
-                                                 "batched cold run x the FP64 per-instance round-trip time "
-                                                 "measured at its grid size (200 round trips per size)"}
-    out["speedup_batched_cold_vs_per_instance"] = out["batched_fp32c_cold"]["points_per_s"] / (len(pts) / est)
-    sur = None
-    if weights and os.path.exists(weights):
-        from cuopo_amd.train_surrogate import load
-        try:
-            sur, meta = load(weights)
-        except ValueError as e:
-            out["surrogate_error"] = str(e)
-    if sur is not None:
-        out["surrogate"] = meta
-        # the network's warm start, and the ablation: the mean training target
-        # as the seed of every point
-        for key, use_mean in (("batched_fp32c_warm", False), ("batched_fp32c_mean_seed", True)):
-            warm = sur.warm_fields_mixed([p[1:] for p in pts], [p[0] for p in pts], ap0, seed=11,
-                                         use_mean=use_mean)
-            w = SW.run_warm(pts, warm, precision=P.FP32C)  # the cold start is the fallback
-            out[key], _ = summarise(w)
-            both = (cold_states == 1) & (w["states"] == 1)
-            rel = np.abs(w["power"] - cold_power) / np.maximum(cold_power, 1e-300)
-            out[key]["rel_power_vs_cold_where_both_steady"] = \
-                {"count": int(both.sum()), "median": float(np.median(rel[both])) if both.any() else None,
-                 "max": float(rel[both].max()) if both.any() else None}
-            out[key]["same_finish_state_as_cold"] = int((w["states"] == cold_states).sum())
-            out[key]["speedup_vs_per_instance"] = out[key]["points_per_s"] / (len(pts) / est)
-            out[key]["cold_fallbacks"] = int(w["fallback"].size)
-            arms[key] = (w["states"], w["power"])
-            w["batch"].close()
-            if w["fallback_batch"] is not None:
-                w["fallback_batch"].close()
-    out["fp64_cold_check"] = fp64_steady_check(pts, arms)
-    return out
+    r = SW.run_batched(pts, precision=P.FP32C)
+    r["batch"].close()
+    st = r["states"]
+    rts = np.where(r["rts"] > 0, r["rts"], 5000)
+    return {"workload": "256 CW three-equation degenerate MgO:PPLN points, N in 2^12..2^15, 100 slices, FP32C, "
+                        "from 1e-20 noise, cap 5000",
+            "run_s": r["run_s"], "steady": int((st == 1).sum()), "off": int((st == 2).sum()),
+            "capped": int((st == 0).sum()), "diverged": int((st == -1).sum()),
+            "steady_points_per_s": float((st == 1).sum() / r["run_s"]),
+            "finished_points_per_s": float(((st == 1) | (st == 2)).sum() / r["run_s"]),
+            "mean_round_trips": float(rts.mean())}, (pts, rts)
 
 
-def fp64_steady_check(pts, arms, max_rt=10000, seed=1000, sigma=1e-20, every=50, tol=1e-6):
-    """The sweep target's accuracy check.  The points that finished steady in
-    any FP32C arm run again in FP64 from the cold run's noise (the same seed
-    per point) for max_rt round trips.  From each FP64 power history: the
-    power where the convergence test (|dP| <= tol P over `every` round trips)
-    first passes, P_test, and the power after max_rt round trips, P_end.
-    Reported: how far FP64 itself moves after its test passes, and each FP32C
-    arm's steady power against P_test and against P_end."""
-    import numpy as np
+FAMILY_N, FAMILY_NZ, SWEEP_CAP = 1 << 14, 100, 20000
+
+
+def headline_sweep(weights, count=256, subset=12, seed=31):
+    """The headline: steady points/s of the learned-warm-start FP64 batch on
+    `count` points of the sPPLT family, and of the plain per-instance FP64
+    cold start on `subset` of the same points (every count/subset-th)."""
     import cuopo_amd as P
+    from cuopo_amd import surrogate as S
     from cuopo_amd import sweep as SW
-    idx = sorted({int(k) for st, _ in arms.values() for k in np.flatnonzero(st == 1)})
-    if not idx:
-        return {"points": 0}
-    sub = [pts[k] for k in idx]
-    warm = [(P.noise(seed + k, 0, sigma, pts[k][0]), P.noise(seed + k, 1, sigma, pts[k][0])) for k in idx]
-    b, _ = SW.make_batch(sub, max_rt=max_rt, precision=P.FP64, warm=warm, autotune=True)
-    b.run_ex(max_rt, check_every=0, tol=0.0)
-    hist = np.array([b.power_history(j, max_rt) for j in range(len(sub))])
+    from cuopo_amd.train_surrogate import load
+    pts = S.sample_points(count, seed)  # the network was trained on other seeds
+    N, NZ, cap = FAMILY_N, FAMILY_NZ, SWEEP_CAP
+    sur, meta = load(weights)
+    ds = [S.derive(S.family_args(*p, N=N, NZ=NZ, NRT=cap)) for p in pts]
+    # timed: network inference on the GPU + decoding into fields, then the batch
+    t0 = time.perf_counter()
+    warm = sur.warm_fields(pts, N, [d.Ap0 for d in ds], seed=seed)
+    t_infer = time.perf_counter() - t0
+    sur.net.close()
+    b, _ = S.family_batch(pts, N, NZ, cap, P.FP64, warm=warm)  # upload (PCIe): outside the timed region
+    log("headline: tuning")
+    tuning = tune(b)
+    log("headline: warm batch")
+    t1 = time.perf_counter()
+    rts, st = b.run_ex(cap, check_every=50, tol=1e-6, off_ratio=1e-6)
+    t_run = time.perf_counter() - t1
+    power = b.power()
+    dev_ms = b.last_run_ms()
+    col_us, row_us = b.time_stages(N, 20)
     b.close()
-    p_test = np.full(len(idx), np.nan)
-    for j, h in enumerate(hist):
-        c = h[every - 1::every]
-        ok = np.flatnonzero(np.abs(c[1:] - c[:-1]) <= tol * c[1:])
-        if ok.size:
-            p_test[j] = c[ok[0] + 1]
-    p_end = hist[:, -1]
-    has = np.isfinite(p_test)
-    drift = np.abs(p_test[has] - p_end[has]) / p_end[has]
+    arm_s = t_infer + t_run
+    steady = np.flatnonzero(st == 1)
+    # plain per-instance FP64 cold starts on the subset, one after another
+    log("headline: per-instance FP64 cold starts")
+    idx = list(range(0, count, max(1, count // subset)))[:subset]
+    cold = {"points": [[float(v) for v in pts[k]] for k in idx], "rts": [], "states": [], "wall_s": []}
+    agree = []
+    for j, k in enumerate(idx):
+        o = SW.run_single_fp64(None, max_rt=cap, seed=1000, k=j, args=S.family_args(*pts[k], N=N, NZ=NZ, NRT=cap))
+        cold["rts"].append(int(o["rts"]))
+        cold["states"].append(int(o["state"]))
+        cold["wall_s"].append(float(o["wall_s"]))
+        if o["state"] == 1 and st[k] == 1:
+            agree.append(abs(power[k] - o["power"]) / o["power"])
+        log(f"headline: cold point {j + 1}/{len(idx)}: state {o['state']} after {o['rts']} round trips")
+    cold_s = float(sum(cold["wall_s"]))
+    cold_steady = int(sum(1 for v in cold["states"] if v == 1))
+    warm_rate = float(steady.size / arm_s)
+    cold_rate = cold_steady / cold_s
+    sub_warm_steady = int(sum(1 for k in idx if st[k] == 1))
+    bz = 2 * 3 * N * 16 * count  # one read + one write of every field of the batch: bytes of one stage launch
+    return {
+        "value": warm_rate,
+        "per_instance_fp64_cold": {
+            "points": len(idx), "steady": cold_steady, "off": int(sum(1 for v in cold["states"] if v == 2)),
+            "capped": int(sum(1 for v in cold["states"] if v == 0)), "run_s": cold_s,
+            "steady_points_per_s": cold_rate, "round_trips": cold["rts"],
+            "mean_round_trips": float(np.mean(cold["rts"])),
+            "seconds_per_point": cold_s / len(idx)},
+        "warm_batch": {"points": count, "steady": int(steady.size), "off": int((st == 2).sum()),
+                       "capped": int((st == 0).sum()), "diverged": int((st == -1).sum()),
+                       "inference_s": t_infer, "run_s": t_run, "device_ms": dev_ms,
+                       "mean_round_trips": float(np.mean(np.where(rts > 0, rts, cap))),
+                       "median_round_trips_steady": float(np.median(rts[steady])) if steady.size else None,
+                       "subset_steady": sub_warm_steady, "tuning": tuning},
+        "speedup_vs_per_instance": (warm_rate / cold_rate) if cold_rate > 0 else None,
+        "speedup_lower_bound_if_cold_none": (warm_rate * cold_s) if cold_rate == 0 else None,
+        "warm_vs_cold_power_where_both_steady": {
+            "points": len(agree), "median": float(np.median(agree)) if agree else None,
+            "max": float(np.max(agree)) if agree else None,
+            "within_1e-4": int(sum(1 for v in agree if v <= 1e-4)), "values": agree},
+        "surrogate": {k: meta[k] for k in ("family", "points", "N", "NZ", "steady_states", "epochs", "val")
+                      if k in meta},
+        "cold_subset": cold,
+        "kernel": {"col_us": col_us, "row_us": row_us, "bz": bz},
+    }
 
-    def stats(rel):
-        return {"median": float(np.median(rel)) if rel.size else None, "max": float(rel.max()) if rel.size else None,
-                "within_1e-4": int((rel <= 1e-4).sum())}
 
-    out = {"points": len(idx), "max_rt": max_rt, "fp64_test_passed": int(has.sum()),
-           "fp64_drift_after_test": stats(drift), "arms": {}}
-    for key, (st, pw) in arms.items():
-        js = [j for j, k in enumerate(idx) if st[k] == 1 and has[j]]
-        a = np.array([pw[idx[j]] for j in js])
-        out["arms"][key] = {"points": len(js),
-                            "vs_fp64_at_test": stats(np.abs(a - p_test[js]) / p_test[js]) if js else None,
-                            "vs_fp64_end": stats(np.abs(a - p_end[js]) / p_end[js]) if js else None}
-    return out
-
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--no-sweep", action="store_true")
-    ap.add_argument("--no-shapes", action="store_true", help="skip the N=2^18 and batched FP64 sections")
-    ap.add_argument("--tuning", default=None,
-                    help="col_threads,row_threads,xcd_pack,phase_mode[,col_spt,row_spt] applied instead of autotuning "
-                         "(profiling runs: the profiler perturbs the autotuner's timings)")
-    ap.add_argument("--only", choices=["c5", "c3", "c3f"], default=None,
-                    help="run only the N=2^18 (c5), the batched FP64 (c3) or the batched FP32C (c3f, the "
-                         "sweep's arithmetic) section and print its JSON (for profiling those shapes)")
-    ap.add_argument("--weights", default=os.path.join(ROOT, "cuopo_amd", "data", "surrogate.npz"))
-    a = ap.parse_args()
-    if a.gpus != 1:
-        print(json.dumps({"error": "this solver runs one instance per GPU and is benchmarked on one GPU; "
-                                   "--gpus above 1 is not supported (independent sweep points shard "
-                                   "across GPUs without communication)"}))
-        sys.exit(2)
-
-    import numpy as np
+def sweep_kernels(weights, count=256, rts=10, seed=31):
+    """The headline batch alone, for profilers: the warm-started FP64 batch
+    of the sPPLT family, tuned, `rts` round trips without the convergence
+    test, then each stage timed alone (20 launches) as the headline does."""
     import cuopo_amd as P
+    from cuopo_amd import surrogate as S
+    from cuopo_amd.train_surrogate import load
+    pts = S.sample_points(count, seed)
+    N, NZ = FAMILY_N, FAMILY_NZ
+    sur, _ = load(weights)
+    warm = sur.warm_fields(pts, N, [S.derive(S.family_args(*p, N=N, NZ=NZ)).Ap0 for p in pts], seed=seed)
+    sur.net.close()
+    b, _ = S.family_batch(pts, N, NZ, rts + 10, P.FP64, warm=warm)
+    tuning = tune(b)
+    t0 = time.perf_counter()
+    b.run(rts, check_every=0, tol=0.0)
+    wall = time.perf_counter() - t0
+    col_us, row_us = b.time_stages(N, 20)
+    b.close()
+    bz = 2 * 3 * N * 16 * count
+    return {"workload": f"{count}-point sPPLT sweep batch, N=2^14, 100 slices, FP64", "round_trips": rts,
+            "round_trips_per_s_per_instance": rts / wall, "col_stage_us": col_us, "row_stage_us": row_us,
+            "col_frac": bz / (col_us * 1e-6) / HBM_PEAK, "row_frac": bz / (row_us * 1e-6) / HBM_PEAK,
+            "tuning": tuning}
 
-    global TUNING
-    if a.tuning:
-        TUNING = tuple(int(v) for v in a.tuning.split(","))
-        if len(TUNING) not in (4, 6):
-            raise SystemExit("--tuning wants col_threads,row_threads,xcd_pack,phase_mode[,col_spt,row_spt]")
-    if a.only:
-        print(json.dumps(c5_section() if a.only == "c5" else c3_section(fp32c=a.only == "c3f")))
-        return
 
+def c2_section(a):
+    """Single-instance round trips/s of the 2^14-sample sPPLT workload (3x
+    threshold, 0.3 rad), plain FP64 cold start, with its kernel roofline."""
+    import cuopo_amd as P
     args = workload_c2(NRT=a.warmup + a.steps + 1)
     inst = P.setup_instance(args, P.FP64, seed=1, noise_sigma=1e-20)
-    d, prm = inst["derived"], inst["params"]
-    s = P.Solver(prm)
+    d = inst["derived"]
+    s = P.Solver(inst["params"])
     s.upload(inst["w"], inst["T"], inst["Ap_in"], inst["Ap"], inst["As"], inst["Ai"])
-    # launch settings chosen by timing the candidates on this instance (the
-    # fields are restored afterwards; setup cost, outside the timed region)
     log("c2: tuning")
     tuning = s.autotune(iters=20) if TUNING is None else tune(s)
     log("c2: running")
-    # warm-up (also builds the round-trip graph used by the timed run)
-    s.run(0, max(a.warmup, 3))
-    # timed region: opo_run launches the graph replays and returns after the
-    # stream's end event has completed (synchronous on both sides)
+    s.run(0, max(a.warmup, 3))  # warm-up (also builds the round-trip graph)
     t0 = time.perf_counter()
-    s.run(a.warmup, a.steps)
+    s.run(a.warmup, a.steps)  # graph replays; returns after the stream's end event
     wall = time.perf_counter() - t0
     dev_ms = s.last_run_ms()
     rts = a.steps / wall
@@ -452,80 +454,117 @@ def main():
     down_s = time.perf_counter() - t1
     if not (np.isfinite(As).all() and np.isfinite(Ap).all()):
         raise SystemExit("non-finite fields after the timed run")
-    # the C ABI takes host buffers: one upload (host-side phase tables + H2D)
-    # and one download (D2H) around the timed run give the PCIe-inclusive rate
     t1 = time.perf_counter()
     s.upload(inst["w"], inst["T"], inst["Ap_in"], inst["Ap"], inst["As"], inst["Ai"])
     up_s = time.perf_counter() - t1
-
-    # roofline of the dominant kernel: per-launch times of the slice's two stages
     col_us, row_us = s.time_stages(200)
+    s.close()
     N, nf, sc = d.N, 3, 16
-    Bz = 2 * nf * N * sc  # one read + one write of every field: algorithmic bytes of one slice
+    Bz = 2 * nf * N * sc
     dom, dom_us = ("col_stage", col_us) if col_us >= row_us else ("row_stage", row_us)
-    achieved = Bz / (dom_us * 1e-6)
-    traffic = load_traffic(f"c2_{dom}")
     copy_gbps, _ = P.copy_probe(256 << 20, 20)
-    # one copy launch that reads and writes the same Bz bytes, timed like the
-    # stages: what launch, fill and drain alone cost at this size
     _, copy_bz_us = P.copy_probe(nf * N * sc, 200)
+    return {"workload": "CW sPPLT three-equation OPO, N=2^14 complex128 samples, 100 z-slices, 20 mm crystal, "
+                        "pump 3x threshold, detuning 0.3 rad, 532->1000+1137 nm, one instance",
+            "round_trips_per_s": rts, "ms_per_round_trip": 1e3 * wall / a.steps, "steps": a.steps,
+            "device_ms_timed_region": dev_ms, "col_stage_us": col_us, "row_stage_us": row_us,
+            "roofline": {"kernel": dom, "kernel_us": dom_us, "achieved_GBps": Bz / (dom_us * 1e-6) / 1e9,
+                         "frac": Bz / (dom_us * 1e-6) / HBM_PEAK, "traffic": load_traffic(f"c2_{dom}")},
+            "graph_slice_us": 1e3 * dev_ms / (a.steps * args.NZ),
+            "copy_probe_GBps": copy_gbps, "copy_same_bytes_us": copy_bz_us, "tuning": tuning,
+            "upload_ms": 1e3 * up_s, "download_ms": 1e3 * down_s,
+            "round_trips_per_s_incl_pcie": a.steps / (wall + up_s + down_s),
+            "round_trip_frac_of_hbm": b_rt(N, args.NZ, nf, 2, sc) * rts / HBM_PEAK,
+            "algorithmic_fp64_tflops": nf * (10 * math.log2(N) + 156) * N * args.NZ * rts / 1e12}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000, help="timed round trips of the single-instance section")
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-shapes", action="store_true", help="skip the N=2^18 and batched PPLN sections")
+    ap.add_argument("--subset", type=int, default=12, help="points of the per-instance FP64 cold-start baseline")
+    ap.add_argument("--tuning", default=None,
+                    help="col_threads,row_threads,xcd_pack,phase_mode[,col_spt,row_spt] applied instead of autotuning "
+                         "(profiling runs: the profiler perturbs the autotuner's timings)")
+    ap.add_argument("--only", choices=["c5", "c3", "c3f", "c2", "sweep"], default=None,
+                    help="run only the N=2^18 (c5), the batched FP64 (c3) or FP32C (c3f) PPLN section, the "
+                         "single-instance 2^14 section (c2) or the headline batch's kernels (sweep) and print its "
+                         "JSON (for profiling those shapes)")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "cuopo_amd", "data", "surrogate.npz"))
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this solver runs one instance per GPU and is benchmarked on one GPU; "
+                                   "--gpus above 1 is not supported (independent sweep points shard "
+                                   "across GPUs without communication)"}))
+        sys.exit(2)
+
+    global TUNING
+    if a.tuning:
+        TUNING = tuple(int(v) for v in a.tuning.split(","))
+        if len(TUNING) not in (4, 6):
+            raise SystemExit("--tuning wants col_threads,row_threads,xcd_pack,phase_mode[,col_spt,row_spt]")
+    if a.only:
+        fn = {"c5": c5_section, "c3": c3_section, "c3f": lambda: c3_section(fp32c=True), "c2": lambda: c2_section(a),
+              "sweep": lambda: sweep_kernels(a.weights)}
+        print(json.dumps(fn[a.only](), default=jsonable))
+        return
+
+    h = headline_sweep(a.weights, subset=a.subset)
+    k = h.pop("kernel")
+    dom, dom_us = ("col_stage", k["col_us"]) if k["col_us"] >= k["row_us"] else ("row_stage", k["row_us"])
+    achieved = k["bz"] / (dom_us * 1e-6)
+    value = h.pop("value")
     out = {
-        "metric": "round trips/s (plain FP64, one instance, cold start)",
-        "value": rts,
-        "unit": "round trips/s",
+        "metric": "sweep points/s to steady state (learned warm start + batched FP64, vs plain per-instance FP64 "
+                  "cold start)",
+        "value": value,
+        "unit": "sweep points/s",
         "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": 1e3 * wall / a.steps,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": 1e3 / value if value > 0 else None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded 1e-20 complex Gaussian noise on signal and idler, CW pump)",
-        "config": {"workload": "CW sPPLT three-equation OPO, N=2^14 complex128 samples, 100 z-slices, "
-                               "20 mm crystal, pump 3x threshold, detuning 0.3 rad, 532->1000+1137 nm",
-                   "N": N, "NZ": args.NZ, "fields": nf, "round_trips_timed": a.steps},
+        "data": "synthetic (sweep points drawn from a seeded generator; cold starts from seeded 1e-20 complex "
+                "Gaussian noise; random-init-free: the network's weights are the committed trained ones)",
+        "config": {"workload": "256-point CW three-equation sPPLT sweep (532 -> 1000 + 1137 nm; pump 1.2-5x "
+                               "threshold, detuning -pi..pi rad, crystal 5-40 mm, GDD -500..500 fs^2, R 0.90-0.99), "
+                               "N=2^14 complex128 samples, 100 z-slices, one batch from the learned warm start to "
+                               "the steady test (|dP| < 1e-6 P over 50 round trips), cap 20000 round trips",
+                   "points": 256, "N": FAMILY_N, "NZ": FAMILY_NZ, "cap": SWEEP_CAP,
+                   "per_instance_subset": a.subset},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic,
-                     "kernel": dom, "kernel_us": dom_us,
-                     "algorithmic_bytes_per_launch": Bz},
-        "extra": {"device_ms_timed_region": dev_ms, "col_stage_us": col_us, "row_stage_us": row_us,
-                  "slice_frac_of_hbm": Bz / ((col_us + row_us) * 1e-6) / HBM_PEAK,
-                  # one slice inside the graph replay of the timed region (both
-                  # stages back to back, plus the round trip's boundary and tick)
-                  "graph_slice_us": 1e3 * dev_ms / (a.steps * args.NZ),
-                  "graph_slice_frac_of_hbm": 2 * Bz / (1e-3 * dev_ms / (a.steps * args.NZ)) / HBM_PEAK,
-                  "copy_probe_GBps": copy_gbps, "tuning": tuning,
-                  "copy_same_bytes_us": copy_bz_us,
-                  "upload_ms": 1e3 * up_s, "download_ms": 1e3 * down_s,
-                  "round_trips_per_s_incl_pcie": a.steps / (wall + up_s + down_s),
-                  "copy_same_bytes_frac_of_hbm": Bz / (copy_bz_us * 1e-6) / HBM_PEAK,
-                  # whole round trip: B_rt x round trips/s against the HBM peak and the
-                  # measured copy rate (a two-pass slice moves every field twice, so
-                  # this fraction is at most half the per-launch one)
-                  "round_trip_frac_of_hbm": b_rt(N, args.NZ, nf, 2, sc) * rts / HBM_PEAK,
-                  "round_trip_frac_of_copy": b_rt(N, args.NZ, nf, 2, sc) * rts / (copy_gbps * 1e9),
-                  # secondary: nf (10 log2 N + 156) flops per sample and slice (FFT pair,
-                  # NL step, linear phase) at the measured rate, against the FP64 peak
-                  "algorithmic_fp64_tflops": nf * (10 * math.log2(N) + 156) * N * args.NZ * rts / 1e12,
-                  "intracavity_power_end": float(np.sum(np.abs(As) ** 2))},
+                     "frac": achieved / HBM_PEAK, "traffic": load_traffic(f"sweep_fp64_{dom}"),
+                     "kernel": dom, "kernel_us": dom_us, "algorithmic_bytes_per_launch": k["bz"]},
+        "extra": {"sweep": h},
     }
+    out["extra"]["c2_single_fp64"] = c2_section(a)
+    c3_cold = None
     if not a.no_shapes:
         out["extra"]["c5_single_fp64"] = c5_section()
         out["extra"]["c3_batched_fp64"] = c3_section()
-    if not a.no_sweep:
-        log("sweep")
-        out["extra"]["sweep"] = sweep_section(a.weights)
+        log("c3 sweep from noise")
+        out["extra"]["c3_sweep_fp32c_cold"], c3_cold = c3_sweep_cold()
     if not a.no_cpu:
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(args, a.cpu_seconds)
-        sweep_b = out["extra"].get("sweep", {}).get("batched_fp32c_cold")
-        log("cpu baselines: C1, C5, sweep on all cores")
-        out["extra"]["cpu"] = cpu_extra(sweep=SWEEP_RTS[0] if SWEEP_RTS else None)
-        c3 = out["extra"]["cpu"]["c3_sweep_all_cores"]
-        if sweep_b and "est_sweep_points_per_s" in c3:
-            c3["gpu_batched_fp32c_over_all_cores"] = sweep_b["points_per_s"] / c3["est_sweep_points_per_s"]
-    print(json.dumps(out))
+        out["cpu_baseline"] = cpu_headline(h["cold_subset"])
+        log("cpu baselines: C2 one core, C1, C5, PPLN sweep on all cores")
+        out["extra"]["cpu"] = cpu_extra(c3_cold=c3_cold)
+        out["extra"]["cpu"]["c2_1core"] = cpu_c2_1core(workload_c2(), a.cpu_seconds)
+    print(json.dumps(out, default=jsonable))
+
+
+def jsonable(o):
+    """numpy scalars and arrays in the output line"""
+    if hasattr(o, "tolist"):
+        return o.tolist()
+    raise TypeError(f"{type(o).__name__} is not JSON serialisable")
 
 
 if __name__ == "__main__":
