@@ -138,3 +138,25 @@ def test_handles_close_on_leaving_a_with_block(gpu):
             b.upload(k, w, T, Ap_in, Ap, As, Ai)
         b.run(2, check_every=0, tol=0.0)
     assert b._h is None
+
+
+def test_full_pulsed_workload_grid_against_oracle(gpu):
+    """The N = 2^18 nanosecond workload at its full 200 slices per pass, EOM
+    and GDD on, all NRT = 3 round trips: every pump slot is read, the last
+    one shifted by a sample (src/headers/functions.cuh:191-213), and every
+    round trip is saved (src/cuOPO.cu:570-608)."""
+    cfg = c5(N=1 << 18, NZ=200, NRT=3)
+    p, T, w, Ap_in, Ap, As, Ai = inputs(cfg, seed=8)
+    As, Ai = grown(As, Ai)
+    s = solver(gpu, cfg)
+    s.upload(w, T, Ap_in, Ap, As, Ai)
+    s.enable_save(True)
+    s.run(0, 3)
+    gp, gs, gi = s.download()
+    sv = s.download_saved()
+    s.close()
+    osv, _ = O.run(p, w, T, Ap_in, Ap, As, Ai, 0, 3, save=True)
+    for g, o in ((gp, Ap), (gs, As), (gi, Ai)):
+        assert rel_l2(g, o) < 1e-9
+    for k in ("p", "s", "i"):
+        assert rel_l2(sv[k], osv[k]) < 1e-9, k
