@@ -3,6 +3,8 @@
     python -m cuopo_amd.train_surrogate --points 13312 \
         --dataset-out cuopo_amd/data/surrogate_dataset.npz --out cuopo_amd/data/surrogate.npz
     python -m cuopo_amd.train_surrogate --dataset cuopo_amd/data/surrogate_dataset.npz   # retrain only
+    python -m cuopo_amd.train_surrogate --dataset cuopo_amd/data/surrogate_dataset.npz --append 3328 \
+        --dataset-out cuopo_amd/data/surrogate_dataset.npz                             # more points, retrain
 
 Steady states come from batched FP32-complex cold starts (seeded 1e-20 noise,
 as the workload starts) of random points of the three-equation sPPLT family
@@ -65,6 +67,8 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--dataset", default="", help="train on this dataset instead of generating one")
+    ap.add_argument("--append", type=int, default=0,
+                    help="with --dataset: run this many more points (seed + 1) and add their steady states")
     ap.add_argument("--dataset-out", default="")
     ap.add_argument("--out", default=os.path.join(HERE, "data", "surrogate.npz"))
     a = ap.parse_args(argv)
@@ -73,6 +77,21 @@ def main(argv=None):
         z = np.load(a.dataset, allow_pickle=False)
         ds = {k: z[k] for k in z.files}
         gen = json.loads(str(ds.pop("meta")))
+        if a.append > 0:
+            seed2 = gen.get("seeds", [gen["seed"]])[-1] + 1
+            more = S.generate_dataset(a.append, seed=seed2, chunk=a.chunk, N=gen["N"], NZ=gen["NZ"],
+                                      max_rt=gen["max_rt"], noise_sigma=gen["noise"], precision=FP32C,
+                                      log=lambda n, c: print(f"dataset: {n} more points run, {c}", flush=True))
+            for k, v in more.items():
+                if v.ndim >= 1:
+                    ds[k] = np.concatenate([ds[k], v]) if ds[k].size else v
+            gen["points"] += a.append
+            gen["seeds"] = gen.get("seeds", [gen["seed"]]) + [seed2]
+            gen["dataset_s"] += time.perf_counter() - t0
+            for k in ("steady", "off", "capped", "diverged"):
+                gen[k] += int(more[k])
+            if a.dataset_out:
+                np.savez_compressed(a.dataset_out, meta=json.dumps(gen), **ds)
     else:
         ds = S.generate_dataset(a.points, seed=a.seed, chunk=a.chunk, N=a.N, NZ=a.nz, max_rt=a.max_rt,
                                 noise_sigma=a.noise, precision=FP32C,
@@ -83,6 +102,7 @@ def main(argv=None):
         if a.dataset_out:
             np.savez_compressed(a.dataset_out, meta=json.dumps(gen),
                                 **{k: v for k, v in ds.items() if v.ndim >= 1})
+        ds = {k: v for k, v in ds.items() if v.ndim >= 1}
     m = len(ds["scale"])
     print(f"dataset: {m} steady states of {gen['points']} points ({gen})", flush=True)
     X = ds["X"].astype(np.float32)

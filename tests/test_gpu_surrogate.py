@@ -50,8 +50,8 @@ def test_warm_start_saves_round_trips_and_reaches_the_cold_state(gpu):
     print(f"cold: {int((cold['states'] == 1).sum())} steady of {len(pts)}; {stats}")
     net = stats["network"]
     assert net["both"] >= 16
-    assert net["rts"] < 0.2 * net["cold_rts"]  # at least 5x fewer round trips (measured ~10x)
-    # measured: the cold start's mode in ~0.7 of the points, median power 5e-5 from the cold start's
+    assert net["rts"] < 0.2 * net["cold_rts"]  # at least 5x fewer round trips (measured 8.4x)
+    # measured: the cold start's mode in 0.63 of the points, median power 4.4e-5 from the cold start's
     assert net["same_mode"] >= 0.5 and net["rel_power_median"] < 1e-3
-    # the network's gain over the ablation is the mode it picks (measured 0.69 vs 0.57)
+    # the network's gain over the ablation is the mode it picks (measured 0.63 vs 0.57)
     assert stats["mean"]["same_mode"] < net["same_mode"]
