@@ -95,6 +95,9 @@ struct Group {
   int *d_rt = nullptr, *d_active = nullptr, *d_conv = nullptr, *d_status = nullptr;
   double *d_partials = nullptr, *d_pref = nullptr, *d_pfirst = nullptr, *d_rprev = nullptr, *d_hist = nullptr;
   int hist_len = 0;
+  // instances the stage and tick launches cover: the first launch_n descriptors of d_insts (all
+  // members after sync_insts; the still-active ones after a compaction, see compact_groups)
+  int launch_n = 0;
   int ct = 0, rt = 0;  // column tile (columns per block), row tile (rows per block)
   int col_threads = 0, row_threads = 0;  // per-group overrides (0 = engine setting)
   int xcd_pack = -1;                      // per-group override (-1 = engine setting)
@@ -119,6 +122,7 @@ struct Engine {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool use_graph = true;
+  bool compact = true;  // shrink the launches to the still-active instances during a converging run
   bool save_enabled = false;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
@@ -469,34 +473,40 @@ int upload_instance(Engine &E, Group &g, int k, const double *w, const double *T
   return OPO_OK;
 }
 
+// Device descriptor of instance k of group g (pointers into the group's
+// per-instance arrays, so its place in d_insts does not matter).
+OpoInst make_desc(const Engine &E, const Group &g, int k) {
+  const Instance &I = g.inst[k];
+  const opo_params &p = I.prm;
+  OpoInst d;
+  std::memset(&d, 0, sizeof(d));
+  d.state = I.state; d.lin = I.lin; d.lfac = I.use_fac && group_factored(g) ? I.lfac[eidx(g.re)] : nullptr; d.gdd = I.gdd; d.eom = p.use_eom ? I.eom : nullptr; d.nl = I.nl;
+  d.pump = I.pump; d.pump_len = I.pump_len;
+  for (int f = 0; f < 3; ++f) d.save[f] = E.save_enabled ? I.save[f] : nullptr;
+  d.save_first = p.regime == OPO_CW ? p.NRT - p.Nrts : 0;
+  d.save_slots = p.regime == OPO_CW ? p.Nrts : p.NRT;
+  d.nrt = p.NRT;
+  d.regime = p.regime;
+  d.h = p.dz / 2;
+  d.kp = p.kp; d.ks = p.ks; d.ki = p.ki;
+  d.sqrt_rs = std::sqrt(p.Rs); d.sqrt_ri = std::sqrt(p.Ri);  // AddPhase, functions.cuh:145
+  d.mir[0] = std::cos(kPi * p.delta); d.mir[1] = std::sin(kPi * p.delta);
+  d.rt = g.d_rt + k; d.active = g.d_active + k; d.partials = g.d_partials + (size_t)k * g.N2;
+  d.power_hist = g.d_hist + (size_t)k * g.hist_len; d.hist_len = g.hist_len;
+  d.conv_rt = g.d_conv + k; d.p_ref = g.d_pref + k; d.p_first = g.d_pfirst + k; d.r_prev = g.d_rprev + k; d.status = g.d_status + k;
+  d.snap = I.snap;
+  d.active_v = 1;
+  return d;
+}
+
 int sync_insts(Engine &E) {
   if (!E.inst_dirty) return OPO_OK;
   for (auto &g : E.groups) {
     const int n = (int)g.members.size();
     std::vector<OpoInst> h(n);
-    for (int k = 0; k < n; ++k) {
-      Instance &I = g.inst[k];
-      const opo_params &p = I.prm;
-      OpoInst &d = h[k];
-      std::memset(&d, 0, sizeof(d));
-      d.state = I.state; d.lin = I.lin; d.lfac = I.use_fac && group_factored(g) ? I.lfac[eidx(g.re)] : nullptr; d.gdd = I.gdd; d.eom = p.use_eom ? I.eom : nullptr; d.nl = I.nl;
-      d.pump = I.pump; d.pump_len = I.pump_len;
-      for (int f = 0; f < 3; ++f) d.save[f] = E.save_enabled ? I.save[f] : nullptr;
-      d.save_first = p.regime == OPO_CW ? p.NRT - p.Nrts : 0;
-      d.save_slots = p.regime == OPO_CW ? p.Nrts : p.NRT;
-      d.nrt = p.NRT;
-      d.regime = p.regime;
-      d.h = p.dz / 2;
-      d.kp = p.kp; d.ks = p.ks; d.ki = p.ki;
-      d.sqrt_rs = std::sqrt(p.Rs); d.sqrt_ri = std::sqrt(p.Ri);  // AddPhase, functions.cuh:145
-      d.mir[0] = std::cos(kPi * p.delta); d.mir[1] = std::sin(kPi * p.delta);
-      d.rt = g.d_rt + k; d.active = g.d_active + k; d.partials = g.d_partials + (size_t)k * g.N2;
-      d.power_hist = g.d_hist + (size_t)k * g.hist_len; d.hist_len = g.hist_len;
-      d.conv_rt = g.d_conv + k; d.p_ref = g.d_pref + k; d.p_first = g.d_pfirst + k; d.r_prev = g.d_rprev + k; d.status = g.d_status + k;
-      d.snap = I.snap;
-      d.active_v = 1;
-    }
+    for (int k = 0; k < n; ++k) h[k] = make_desc(E, g, k);
     HIPCHK(hipMemcpyAsync(g.d_insts, h.data(), n * sizeof(OpoInst), hipMemcpyHostToDevice, E.stream));
+    g.launch_n = n;
   }
   HIPCHK(hipStreamSynchronize(E.stream));
   E.inst_dirty = false;
@@ -524,7 +534,7 @@ int emit_col(Engine &E, Group &g, unsigned ops, int load, int inv, int fwd, int 
     a.d_kp = I.prm.kp; a.d_ks = I.prm.ks; a.d_h = I.prm.dz / 2;
   }
   const int threads_ct = g.ct;
-  int e = opo::launch_col(g.fp32, g.N1, g.NF, threads_ct, (int)g.members.size(), g.d_insts, a, g.cur, e0, e1);
+  int e = opo::launch_col(g.fp32, g.N1, g.NF, threads_ct, g.launch_n, g.d_insts, a, g.cur, e0, e1);
   if (e != 0) return fail(OPO_ERR_HIP, std::string("col_stage launch: ") + hipGetErrorString((hipError_t)e));
   return OPO_OK;
 }
@@ -546,14 +556,14 @@ int emit_row(Engine &E, Group &g, bool gdd, int mask, hipEvent_t e0 = nullptr, h
     a.d_state = I.state; a.d_lin = I.lin; a.d_gdd = I.gdd;
     a.d_lfac = I.use_fac && group_factored(g) ? I.lfac[eidx(g.re)] : nullptr;
   }
-  int e = opo::launch_row(g.fp32, g.N2, g.NF, g.rt, (int)g.members.size(), 0, g.d_insts, a, g.cur, e0, e1);
+  int e = opo::launch_row(g.fp32, g.N2, g.NF, g.rt, g.launch_n, 0, g.d_insts, a, g.cur, e0, e1);
   if (e != 0) return fail(OPO_ERR_HIP, std::string("row_stage launch: ") + hipGetErrorString((hipError_t)e));
   return OPO_OK;
 }
 
 int emit_tick(Engine &E, Group &g, int check_every, double tol) {
   opo::TickArgs a{};
-  a.count = (int)g.members.size(); a.tiles = g.N2 / g.ct; a.check_every = check_every; a.tol = tol;
+  a.count = g.launch_n; a.tiles = g.N2 / g.ct; a.check_every = check_every; a.tol = tol;
   a.off_ratio = E.off_ratio;
   int e = opo::launch_tick(g.d_insts, a, g.cur);
   if (e != 0) return fail(OPO_ERR_HIP, std::string("tick launch: ") + hipGetErrorString((hipError_t)e));
@@ -665,14 +675,34 @@ int build_graph(Engine &E, int check_every, double tol) {
   return OPO_OK;
 }
 
-int any_active(Engine &E, bool *out) {
-  *out = false;
+// After a convergence check: *alive = some instance still iterates.  With
+// compaction on, a group whose active instances have fallen to half of its
+// launch (or fewer) gets a descriptor array of just those, and its launches
+// shrink to them (*changed); a group with none left keeps one inactive
+// descriptor, whose blocks return at once.  Finished instances were skipped
+// by every block anyway, so the arithmetic and the results do not change.
+int check_active(Engine &E, bool *alive, bool *changed) {
+  *alive = false;
+  *changed = false;
   for (auto &g : E.groups) {
     const int n = (int)g.members.size();
     std::vector<int> act(n);
     HIPCHK(hipMemcpyAsync(act.data(), g.d_active, n * sizeof(int), hipMemcpyDeviceToHost, E.stream));
     HIPCHK(hipStreamSynchronize(E.stream));
-    for (int v : act) if (v) { *out = true; return OPO_OK; }
+    std::vector<OpoInst> h;
+    for (int k = 0; k < n; ++k)
+      if (act[k]) h.push_back(make_desc(E, g, k));
+    if (!h.empty()) *alive = true;
+    const int m = (int)h.size();
+    if (!E.compact || g.launch_n <= 1 || 2 * m > g.launch_n) continue;
+    if (m == 0) {
+      for (int k = 0; k < n && h.empty(); ++k) h.push_back(make_desc(E, g, k));
+      h[0].active_v = 0;
+    }
+    HIPCHK(hipMemcpyAsync(g.d_insts, h.data(), h.size() * sizeof(OpoInst), hipMemcpyHostToDevice, E.stream));
+    HIPCHK(hipStreamSynchronize(E.stream));
+    g.launch_n = (int)h.size();
+    *changed = true;
   }
   return OPO_OK;
 }
@@ -697,7 +727,7 @@ int engine_run(Engine &E, int rt0, int n_rt, int check_every, double tol) {
   for (auto &g : E.groups) RCHK(emit_start(E, g));
   RCHK(join_groups(E));
   int done = 0;
-  bool alive = true;
+  bool alive = true, compacted = false;
   while (done < n_rt - 1 && alive) {
     const int seg = check_every > 0 ? std::min(check_every, n_rt - 1 - done) : n_rt - 1 - done;
     for (int r = 0; r < seg; ++r) {
@@ -710,7 +740,15 @@ int engine_run(Engine &E, int rt0, int n_rt, int check_every, double tol) {
       }
     }
     done += seg;
-    if (check_every > 0) RCHK(any_active(E, &alive));
+    if (check_every > 0) {
+      bool changed = false;
+      RCHK(check_active(E, &alive, &changed));
+      if (changed) {
+        compacted = true;
+        E.graph_key = -1;  // the captured launches cover the old instance count
+        if (graph && alive && done < n_rt - 1) RCHK(build_graph(E, check_every, tol));
+      }
+    }
   }
   if (alive) {
     RCHK(fork_groups(E));
@@ -725,6 +763,11 @@ int engine_run(Engine &E, int rt0, int n_rt, int check_every, double tol) {
     HIPCHK(hipStreamSynchronize(E.stream));
   }
   E.last_rt0 = rt0; E.last_nrt = n_rt;
+  if (compacted) {
+    // the next run starts from every instance again: full descriptors, new capture
+    E.inst_dirty = true;
+    E.graph_key = -1;
+  }
   // surface a launch-time fault immediately
   HIPCHK(hipGetLastError());
   return OPO_OK;
@@ -743,6 +786,8 @@ int engine_init(Engine &E, const opo_params *params, int count) {
   // OPO_USE_GRAPH=0 launches every stage eagerly (profilers that cannot
   // trace graph-launched kernels); results are identical either way.
   if (const char *ug = std::getenv("OPO_USE_GRAPH")) E.use_graph = ug[0] != '0';
+  // OPO_COMPACT=0 keeps every instance in every launch of a converging batch run
+  if (const char *cp = std::getenv("OPO_COMPACT")) E.compact = cp[0] != '0';
   E.count = count; E.NZ = p0.NZ; E.three_eqs = p0.three_eqs ? 1 : 0; E.regime = p0.regime;
   E.fp32 = p0.precision == OPO_FP32C; E.device = p0.device;
   E.NRT = 1;

@@ -295,3 +295,54 @@ def test_invalid_calls_fail_cleanly(gpu):
         s.nl_half(99, 1)
     with pytest.raises(P.OpoError):
         s.set_tiles(48, 0)
+
+
+def test_compaction_leaves_results_unchanged(gpu, monkeypatch):
+    """A converging batch shrinks its launches to the still-active instances
+    (OPO_COMPACT, on by default); instances that finished were skipped by
+    every block before, so round trips, states, powers, power histories and
+    fields equal those of a batch that keeps every instance in every launch,
+    bit for bit.  A second run on the same batch starts from all instances."""
+    P = gpu
+    cfgs = []
+    for N in (1024, 2048):
+        for nth in (0.5, 0.8, 2.0, 5.0, 10.0, 20.0):
+            c = c2(N=N, NZ=4, NRT=2000)
+            c.Nth = nth
+            cfgs.append(c)
+    made = [instance(P, c, 40 + k, 1e-6) for k, c in enumerate(cfgs)]
+
+    def fresh():
+        b = P.Batch([m[0] for m in made])
+        for k, (_, x) in enumerate(made):
+            b.upload(k, x["w"], x["T"], x["Ap_in"], x["Ap"], x["As"], x["Ai"])
+        return b
+
+    def result(b):
+        rts, st = b.run_ex(2000, check_every=50, tol=1e-3, off_ratio=1e-6)
+        hist = [b.power_history(k, 2000) for k in range(len(cfgs))]
+        return rts.tolist(), st.tolist(), b.power().tolist(), hist, [b.download(k) for k in range(len(cfgs))]
+
+    def same(r1, r2):
+        assert r1[:3] == r2[:3]
+        for h1, h2 in zip(r1[3], r2[3]):
+            assert np.array_equal(h1, h2)
+        for f1, f2 in zip(r1[4], r2[4]):
+            for a, c in zip(f1, f2):
+                assert (a is None and c is None) or np.array_equal(a, c)
+
+    b = fresh()
+    on = result(b)
+    for k, (_, x) in enumerate(made):  # the same batch again, from the same inputs
+        b.upload(k, x["w"], x["T"], x["Ap_in"], x["Ap"], x["As"], x["Ai"])
+    again = result(b)
+    b.close()
+    monkeypatch.setenv("OPO_COMPACT", "0")  # read when the batch is created
+    e = fresh()
+    off = result(e)
+    e.close()
+    rts, st = on[0], on[1]
+    assert 1 in st and 2 in st  # some steady, some off: instances finish at different round trips
+    assert min(r for r in rts if r > 0) < max(rts)
+    same(on, off)
+    same(on, again)
