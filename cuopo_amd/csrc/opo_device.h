@@ -103,11 +103,29 @@ template <int R, int S, typename V> __device__ __forceinline__ void dftr(V* u) {
 
 template <bool PAD> __device__ __forceinline__ int lidx(int e, int es) { return PAD ? e + (e >> 3) : e * es; }
 
-// per-thread twiddles of passes 1 .. Q-1 (pass 0 has none)
-constexpr int tw_off(int M, int Q, int E) {
-  return Q <= 1 ? 0 : tw_off(M, Q - 1, E) + (radix_at(M, Q - 1, E) - 1) * (E / radix_at(M, Q - 1, E));
+// Per-thread twiddle registers of passes 1 .. Q-1 (pass 0 has none).  FP64
+// (`pow`) keeps one root per butterfly, w = e^{2 pi i k/(P R)}, and forms the
+// other legs' roots w^2 .. w^(R-1) by products (at most three deep, so a few
+// ulp, ~1e-15): fewer live registers, and the row stage holds more waves per
+// SIMD.  FP32 keeps every leg's root as looked up, since the same product
+// rounding would be 1e-7 there.
+constexpr int tw_off(int M, int Q, int E, bool pow) {
+  return Q <= 1 ? 0
+                : tw_off(M, Q - 1, E, pow) +
+                      (pow ? 1 : radix_at(M, Q - 1, E) - 1) * (E / radix_at(M, Q - 1, E));
 }
-constexpr int tw_count(int M, int E) { return tw_off(M, npasses(M, E), E); }
+constexpr int tw_count(int M, int E, bool pow) { return tw_off(M, npasses(M, E), E, pow); }
+template <typename V> constexpr bool tw_pow() { return sizeof(V) == 16; }  // double2
+
+// u[j] *= w^j for j = 1 .. R-1, the powers by squaring
+template <int R, typename V> __device__ __forceinline__ void mul_root_powers(V *u, V w) {
+  V p[R > 1 ? R : 2];
+  p[1] = w;
+  if constexpr (R > 2) { p[2] = cmul(w, w); p[3] = cmul(p[2], w); }
+  if constexpr (R > 4) { p[4] = cmul(p[2], p[2]); p[5] = cmul(p[4], w); p[6] = cmul(p[4], p[2]); p[7] = cmul(p[4], p[3]); }
+#pragma unroll
+  for (int j = 1; j < R; ++j) u[j] = cmul(u[j], p[j]);
+}
 
 // The single-wavefront rule, shared by the kernels and by the launchers that
 // size their LDS (one exchange buffer when it holds, two otherwise): a team
@@ -131,7 +149,8 @@ __device__ __forceinline__ void fft_pass(V *v, bool on, V *buf0, int &ph, int t,
   constexpr int T = M / E;
   constexpr int NB = E / R;
   constexpr bool LAST = Q == npasses(M, E) - 1;
-  constexpr int TOFF = tw_off(M, Q, E);  // constant: keeps the register set statically indexed
+  constexpr bool POW = TREG && tw_pow<V>();
+  constexpr int TOFF = tw_off(M, Q, E, POW);  // constant: keeps the register set statically indexed
   V *buf = wave ? buf0 : buf0 + (ph & 1) * bufstride;
   if (on) {
 #pragma unroll
@@ -142,10 +161,14 @@ __device__ __forceinline__ void fft_pass(V *v, bool on, V *buf0, int &ph, int t,
 #pragma unroll
       for (int j = 0; j < R; ++j) u[j] = v[b + j * NB];
       if constexpr (P > 1) {
+        if constexpr (POW) {
+          mul_root_powers<R>(u, tw_sign<S>(tws[TOFF + b]));
+        } else {
 #pragma unroll
-        for (int j = 1; j < R; ++j) {
-          if constexpr (TREG) u[j] = cmul(u[j], tw_sign<S>(tws[TOFF + b * (R - 1) + j - 1]));
-          else u[j] = cmul(u[j], tw_sign<S>(tws[j * k * (M / (P * R))]));
+          for (int j = 1; j < R; ++j) {
+            if constexpr (TREG) u[j] = cmul(u[j], tw_sign<S>(tws[TOFF + b * (R - 1) + j - 1]));
+            else u[j] = cmul(u[j], tw_sign<S>(tws[j * k * (M / (P * R))]));
+          }
         }
       }
       dftr<R, S>(u);
@@ -181,7 +204,8 @@ __device__ __forceinline__ void team_fft(V *v, bool on, V *buf0, int &ph, int t,
 }
 
 // This thread's twiddles of every pass, from the M-point root table `g`
-// (global memory), into treg[tw_count(M, E)]; both transform directions use them.
+// (global memory), into treg[tw_count(M, E, tw_pow<V>())]; both transform
+// directions use them.
 template <int M, int E, int Q = 1, typename V>
 __device__ __forceinline__ void load_twiddles(V *treg, int t, const V *__restrict__ g) {
   if constexpr (Q < npasses(M, E)) {
@@ -189,12 +213,17 @@ __device__ __forceinline__ void load_twiddles(V *treg, int t, const V *__restric
     constexpr int P = p_at(Q, E);
     constexpr int T = M / E;
     constexpr int NB = E / R;
-    constexpr int TOFF = tw_off(M, Q, E);
+    constexpr bool POW = tw_pow<V>();
+    constexpr int TOFF = tw_off(M, Q, E, POW);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const int k = (t + b * T) & (P - 1);
+      if constexpr (POW) {
+        treg[TOFF + b] = g[k * (M / (P * R))];
+      } else {
 #pragma unroll
-      for (int j = 1; j < R; ++j) treg[TOFF + b * (R - 1) + j - 1] = g[j * k * (M / (P * R))];
+        for (int j = 1; j < R; ++j) treg[TOFF + b * (R - 1) + j - 1] = g[j * k * (M / (P * R))];
+      }
     }
     load_twiddles<M, E, Q + 1>(treg, t, g);
   }

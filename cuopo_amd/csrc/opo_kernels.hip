@@ -408,7 +408,8 @@ __global__ __launch_bounds__(row_max_threads(N2, E)) void row_stage(const OpoIns
   const long long N = (long long)a.N1 * N2;
   V *row = reinterpret_cast<V *>(DIRECT ? a.d_state : I->state) + f * N + (long long)N2 * k1;
 
-  V v[E], ph_tab[E], treg[tw_count(N2, E) > 0 ? tw_count(N2, E) : 1];
+  constexpr int NTW = tw_count(N2, E, tw_pow<V>());
+  V v[E], ph_tab[E], treg[NTW > 0 ? NTW : 1];
   const V *fac = reinterpret_cast<const V *>(DIRECT ? a.d_lfac : I->lfac);
   const bool use_fac = !a.use_gdd && fac;
   const V *tab = a.use_gdd ? reinterpret_cast<const V *>(DIRECT ? a.d_gdd : I->gdd) + (f - 1) * N
