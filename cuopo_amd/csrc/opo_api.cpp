@@ -96,9 +96,10 @@ struct Group {
   double *d_partials = nullptr, *d_pref = nullptr, *d_pfirst = nullptr, *d_rprev = nullptr, *d_hist = nullptr;
   int hist_len = 0;
   // instances the stage and tick launches cover: the first launch_n descriptors of d_insts (all
-  // members after sync_insts; the still-active ones after a compaction, see compact_groups)
+  // members after sync_insts; the still-active ones after a compaction, see check_active)
   int launch_n = 0;
   int ct = 0, rt = 0;  // column tile (columns per block), row tile (rows per block)
+  int rt_tuned = 0;    // the row tile to restore after a compacted run narrowed it (0: not narrowed)
   int col_threads = 0, row_threads = 0;  // per-group overrides (0 = engine setting)
   int xcd_pack = -1;                      // per-group override (-1 = engine setting)
   int phase_mode = -1;                    // linear phase: -1 auto, 0 full table, 1 factorised
@@ -260,6 +261,7 @@ void pick_tiles(Engine &E, Group &g) {
   ct = std::min(floor_pow2(ct), g.N2);
   while (ct > 1 && g.NF * ct * T1 > max_threads(g.ce)) ct >>= 1;
   g.ct = ct;
+  g.rt_tuned = 0;
   g.rt = std::max(1, std::min(floor_pow2(std::max(1, rtn / T2)), g.N1));
   while (g.rt > 1 && g.rt * T2 > max_row_threads(g.re, g.N2)) g.rt >>= 1;
   while (g.ct > 1 && col_lds(g, g.ct) > kMaxLds) g.ct >>= 1;
@@ -676,11 +678,15 @@ int build_graph(Engine &E, int check_every, double tol) {
 }
 
 // After a convergence check: *alive = some instance still iterates.  With
-// compaction on, a group whose active instances have fallen to half of its
-// launch (or fewer) gets a descriptor array of just those, and its launches
-// shrink to them (*changed); a group with none left keeps one inactive
-// descriptor, whose blocks return at once.  Finished instances were skipped
-// by every block anyway, so the arithmetic and the results do not change.
+// compaction on, a group in which an instance has finished since its last
+// compaction gets a descriptor array of the active ones, and its launches
+// shrink to them (*changed; the caller captures the round trip again, a few
+// ms against the 50 round trips between checks); a group with none left
+// keeps one inactive descriptor, whose blocks return at once.  Finished
+// instances were skipped by every block anyway, so the arithmetic and the
+// results do not change.
+constexpr int kNarrowRowsBelow = 64;  // launches of this many instances or fewer get one-wavefront row blocks
+
 int check_active(Engine &E, bool *alive, bool *changed) {
   *alive = false;
   *changed = false;
@@ -694,7 +700,7 @@ int check_active(Engine &E, bool *alive, bool *changed) {
       if (act[k]) h.push_back(make_desc(E, g, k));
     if (!h.empty()) *alive = true;
     const int m = (int)h.size();
-    if (!E.compact || g.launch_n <= 1 || 2 * m > g.launch_n) continue;
+    if (!E.compact || g.launch_n <= 1 || m >= g.launch_n) continue;
     if (m == 0) {
       for (int k = 0; k < n && h.empty(); ++k) h.push_back(make_desc(E, g, k));
       h[0].active_v = 0;
@@ -702,14 +708,32 @@ int check_active(Engine &E, bool *alive, bool *changed) {
     HIPCHK(hipMemcpyAsync(g.d_insts, h.data(), h.size() * sizeof(OpoInst), hipMemcpyHostToDevice, E.stream));
     HIPCHK(hipStreamSynchronize(E.stream));
     g.launch_n = (int)h.size();
+    // A few instances leave most CUs idle and every stage latency-bound; rows
+    // of at most one wavefront per block then spread them over more CUs (the
+    // tail of a sweep at 2^14 samples: 2-7% less time per round trip at 1-64
+    // instances).
+    // Only rows are regrouped: every row is transformed exactly as before,
+    // while the column tile also fixes the power partials' summation order.
+    const int one_wave_rt = std::max(1, 64 / (g.N2 / g.re));
+    if (g.launch_n <= kNarrowRowsBelow && g.rt > one_wave_rt) {
+      if (!g.rt_tuned) g.rt_tuned = g.rt;
+      g.rt = one_wave_rt;
+    }
     *changed = true;
   }
   return OPO_OK;
 }
 
+// the tuned row tile again after a run that narrowed it
+void restore_row_tiles(Engine &E) {
+  for (auto &g : E.groups)
+    if (g.rt_tuned) { g.rt = g.rt_tuned; g.rt_tuned = 0; }
+}
+
 // n_rt round trips starting at rt0 (every instance), ending in the time domain.
 int engine_run(Engine &E, int rt0, int n_rt, int check_every, double tol) {
   if (n_rt <= 0) return OPO_OK;
+  restore_row_tiles(E);  // after a compacted run that ended early on an error
   for (auto &g : E.groups) {
     for (auto &I : g.inst)
       if (!I.uploaded) return fail(OPO_ERR_STATE, "run before upload");
@@ -764,9 +788,11 @@ int engine_run(Engine &E, int rt0, int n_rt, int check_every, double tol) {
   }
   E.last_rt0 = rt0; E.last_nrt = n_rt;
   if (compacted) {
-    // the next run starts from every instance again: full descriptors, new capture
+    // the next run starts from every instance again: full descriptors, the
+    // tuned row tiles, new capture
     E.inst_dirty = true;
     E.graph_key = -1;
+    restore_row_tiles(E);
   }
   // surface a launch-time fault immediately
   HIPCHK(hipGetLastError());

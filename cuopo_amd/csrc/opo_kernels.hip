@@ -197,6 +197,7 @@ __global__ __launch_bounds__(E == 4 ? 1024 : 512) void col_stage(const OpoInst *
     if (ops & OP_NL_START) nls = load_nl<NF, V>(I->nl, I->kp, I->ks, I->h, a.s_start);
   }
   V W[E];
+  V x0r{}, b1r{};  // FP64: the four-step roots' base, kept across the pointwise program instead of W
   if (any_fft) {
     // column root table into LDS (the block has NF*FT threads; blockDim
     // would cost a scalar load whose wait covers the NL loads)
@@ -209,6 +210,7 @@ __global__ __launch_bounds__(E == 4 ? 1024 : 512) void col_stage(const OpoInst *
       // grows at most 7x, ~1e-15 in FP64): 4 gathers per thread instead of 2E
       const V x0 = cmul(x0h, x0l);
       const V b1 = cmul(b1h, b1l);
+      x0r = x0; b1r = b1;
       const V b2 = cmul(b1, b1), b4 = cmul(b2, b2);
 #pragma unroll
       for (int m = 0; m < E; ++m) {
@@ -368,6 +370,23 @@ __global__ __launch_bounds__(E == 4 ? 1024 : 512) void col_stage(const OpoInst *
   if (fwd_mask) {
     team_fft<N1, +1, false, false, E>(v, fw, mine0, ph, t, c, CT, bufstride, ltw, wave);
     if (fw) {
+      if constexpr (F64) {
+        // the same W_m again, from the two kept roots: holding 2 complex
+        // values instead of E across the middle of the kernel takes 28 VGPRs
+        // off the E = 8 slice-interior build (154 -> 126, 3 -> 4 waves per
+        // SIMD).  The empty asm hides the recomputation from common
+        // subexpression elimination, which would otherwise keep W alive.
+        asm volatile("" : "+v"(x0r.x), "+v"(x0r.y), "+v"(b1r.x), "+v"(b1r.y));
+        const V b2 = cmul(b1r, b1r), b4 = cmul(b2, b2);
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+          V w = x0r;
+          if (m & 4) w = cmul(w, b4);
+          if (m & 2) w = cmul(w, b2);
+          if (m & 1) w = cmul(w, b1r);
+          W[m] = w;
+        }
+      }
 #pragma unroll
       for (int m = 0; m < E; ++m) v[m] = cmul(v[m], W[m]);
     }
