@@ -10,11 +10,54 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace opo {
 
 template <typename T> struct vec2;
 template <> struct vec2<double> { using type = double2; };
 template <> struct vec2<float> { using type = float2; };
+
+// A device-memory pointer in the global address space.  Pointers read from
+// instance descriptors are generic, and loads and stores through a generic
+// pointer are FLAT operations, which the hardware also counts on the LDS /
+// scalar-memory counter: every later LDS wait of the wave would then also wait
+// for them.  Through a global pointer they are GLOBAL operations (uniform
+// reads become scalar loads), so field loads in flight overlap the transform's
+// LDS exchanges.  Only device-memory (hipMalloc) pointers are cast.
+// Scalars: a plain global pointer.  Complex values (double2 / float2, whose
+// class assignment cannot bind to a global-space reference): a pointer to the
+// native two-element vector of the same layout, indexed through a proxy that
+// loads and stores one 16- (8-) byte element.
+template <typename S> __device__ __forceinline__ __attribute__((address_space(1))) S *gp_scalar(S *p) {
+  return (__attribute__((address_space(1))) S *)p;
+}
+template <typename V> struct native2;
+template <> struct native2<double2> { using type = double __attribute__((ext_vector_type(2))); };
+template <> struct native2<float2> { using type = float __attribute__((ext_vector_type(2))); };
+template <typename V, bool W> struct GPtr {
+  using N = typename native2<V>::type;
+  using P = typename std::conditional<W, __attribute__((address_space(1))) N *,
+                                      const __attribute__((address_space(1))) N *>::type;
+  P p;
+  struct Ref {
+    P q;
+    __device__ __forceinline__ operator V() const { const N r = *q; V v; v.x = r.x; v.y = r.y; return v; }
+    __device__ __forceinline__ const Ref &operator=(V v) const {
+      static_assert(W, "store through a read-only pointer");
+      N r; r.x = v.x; r.y = v.y; *q = r; return *this;
+    }
+  };
+  __device__ __forceinline__ Ref operator[](long long i) const { return Ref{p + i}; }
+  __device__ __forceinline__ GPtr operator+(long long i) const { return GPtr{p + i}; }
+  __device__ __forceinline__ explicit operator bool() const { return p != nullptr; }
+};
+template <typename V> __device__ __forceinline__ GPtr<V, true> gp(V *p) {
+  return GPtr<V, true>{(typename GPtr<V, true>::P)p};
+}
+template <typename V> __device__ __forceinline__ GPtr<V, false> gp(const V *p) {
+  return GPtr<V, false>{(typename GPtr<V, false>::P)p};
+}
 
 template <typename V> __device__ __forceinline__ V mk(decltype(V::x) a, decltype(V::x) b) {
   V r; r.x = a; r.y = b; return r;
@@ -206,8 +249,8 @@ __device__ __forceinline__ void team_fft(V *v, bool on, V *buf0, int &ph, int t,
 // This thread's twiddles of every pass, from the M-point root table `g`
 // (global memory), into treg[tw_count(M, E, tw_pow<V>())]; both transform
 // directions use them.
-template <int M, int E, int Q = 1, typename V>
-__device__ __forceinline__ void load_twiddles(V *treg, int t, const V *__restrict__ g) {
+template <int M, int E, int Q = 1, typename V, typename G>
+__device__ __forceinline__ void load_twiddles(V *treg, int t, G g) {
   if constexpr (Q < npasses(M, E)) {
     constexpr int R = radix_at(M, Q, E);
     constexpr int P = p_at(Q, E);
@@ -219,10 +262,10 @@ __device__ __forceinline__ void load_twiddles(V *treg, int t, const V *__restric
     for (int b = 0; b < NB; ++b) {
       const int k = (t + b * T) & (P - 1);
       if constexpr (POW) {
-        treg[TOFF + b] = g[k * (M / (P * R))];
+        treg[TOFF + b] = V(g[k * (M / (P * R))]);
       } else {
 #pragma unroll
-        for (int j = 1; j < R; ++j) treg[TOFF + b * (R - 1) + j - 1] = g[j * k * (M / (P * R))];
+        for (int j = 1; j < R; ++j) treg[TOFF + b * (R - 1) + j - 1] = V(g[j * k * (M / (P * R))]);
       }
     }
     load_twiddles<M, E, Q + 1>(treg, t, g);

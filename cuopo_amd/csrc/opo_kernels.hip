@@ -25,7 +25,7 @@
 
 namespace opo {
 
-template <typename V> __device__ __forceinline__ V ld_c(const double *p) {
+template <typename V, typename P> __device__ __forceinline__ V ld_c(P p) {
   return mk<V>((decltype(V::x))p[0], (decltype(V::x))p[1]);
 }
 
@@ -54,7 +54,7 @@ template <int NF, typename V> struct NlCoef {
 template <int NF, typename V>
 __device__ __forceinline__ NlCoef<NF, V> load_nl(const double *nl, double kp, double ks, double h, int s) {
   NlCoef<NF, V> r;
-  const double *q = nl + 16 * s;
+  const auto q = gp_scalar(nl) + 16 * s;
 #pragma unroll
   for (int j = 0; j < (NF == 3 ? 3 : 6); ++j) r.c[j] = ld_c<V>(q + 2 * j);
   using R = decltype(V::x);
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(E == 4 ? 1024 : 512) void col_stage(const OpoInst *
   const int n2 = bx * CT + c;
   const long long N = (long long)N1 * N2;
   const unsigned nmask = (unsigned)(N - 1);
-  V *st = reinterpret_cast<V *>(DIRECT ? a.d_state : I->state);
+  const auto st = gp(reinterpret_cast<V *>(DIRECT ? a.d_state : I->state));
   const int fstride = CT * N1;
   const int bufstride = NF * fstride;
   const int inv_mask = MID ? (1 << NF) - 1 : a.inv_mask;
@@ -164,11 +164,11 @@ __global__ __launch_bounds__(E == 4 ? 1024 : 512) void col_stage(const OpoInst *
   V v[E];
   const bool ld = (load_mask >> f) & 1;
 #pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = ld ? st[f * N + (long long)N2 * (t + m * T1) + n2] : mk<V>(0, 0);
+  for (int m = 0; m < E; ++m) v[m] = ld ? V(st[f * N + (long long)N2 * (t + m * T1) + n2]) : mk<V>(0, 0);
   // four-step twiddles W_m = e^{2 pi i n2 k1 / N} for this thread's
   // k1 = t + m*T1, from two-level root tables: X(e) = hi[e >> lobits] lo[e & mask]
-  const V *twlo = reinterpret_cast<const V *>(a.twlo);
-  const V *twhi = reinterpret_cast<const V *>(a.twhi);
+  const auto twlo = gp(reinterpret_cast<const V *>(a.twlo));
+  const auto twhi = gp(reinterpret_cast<const V *>(a.twhi));
   const unsigned lm = (1u << a.lobits) - 1u;
   constexpr bool F64 = std::is_same<T, double>::value;
   // gathered roots: FP64 x0 and the base b1 (hi, lo each); FP32 one pair per m
@@ -201,7 +201,7 @@ __global__ __launch_bounds__(E == 4 ? 1024 : 512) void col_stage(const OpoInst *
   if (any_fft) {
     // column root table into LDS (the block has NF*FT threads; blockDim
     // would cost a scalar load whose wait covers the NL loads)
-    const V *tw1 = reinterpret_cast<const V *>(a.tw1);
+    const auto tw1 = gp(reinterpret_cast<const V *>(a.tw1));
     for (int e = tid; e < N1; e += NF * FT) ltw[e] = tw1[e];
     if (wave) __syncthreads();  // the exchanges no longer order the table's writes
     if constexpr (F64) {
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(E == 4 ? 1024 : 512) void col_stage(const OpoInst *
     if (ops & OP_NL_END) nl_half_pt<NF, 4>(xp, xs, xi, nle);
     if constexpr (!MID) {
       if ((ops & OP_EOM) && I->eom) {
-        const V *eom = reinterpret_cast<const V *>(I->eom);
+        const auto eom = gp(reinterpret_cast<const V *>(I->eom));
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const V e = eom[(long long)N2 * (t + (hb + q) * T1) + n2];
@@ -312,13 +312,13 @@ __global__ __launch_bounds__(E == 4 ? 1024 : 512) void col_stage(const OpoInst *
         V w4[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) w4[q] = pick(f, xp[q], xs[q], xi[q]);  // this group's field
-        V *sv = reinterpret_cast<V *>(I->save[f]);
+        const auto sv = gp(reinterpret_cast<V *>(I->save[f]));
         if ((ops & OP_SAVE) && slot >= 0 && slot < I->save_slots && sv) {
 #pragma unroll
           for (int q = 0; q < 4; ++q) sv[(long long)slot * N + (long long)N2 * (t + (hb + q) * T1) + n2] = w4[q];
         }
         if (snap) {
-          V *sn = reinterpret_cast<V *>(I->snap);
+          const auto sn = gp(reinterpret_cast<V *>(I->snap));
 #pragma unroll
           for (int q = 0; q < 4; ++q) sn[f * N + (long long)N2 * (t + (hb + q) * T1) + n2] = w4[q];
         }
@@ -328,7 +328,7 @@ __global__ __launch_bounds__(E == 4 ? 1024 : 512) void col_stage(const OpoInst *
         }
       }
       if (ops & OP_PUMP) {
-        const V *pin = reinterpret_cast<const V *>(I->pump);
+        const auto pin = gp(reinterpret_cast<const V *>(I->pump));
         long long off = 0;  // functions.cuh:196-210 (NS); CW copies Ap_in (cuOPO.cu:480)
         if (I->regime == 1) {
           if (rp == 0) off = 0;
@@ -425,14 +425,14 @@ __global__ __launch_bounds__(row_max_threads(N2, E)) void row_stage(const OpoIns
   const int r = tid / T2;
   const int k1 = (bx - fi * tiles_per_field) * RT + r;
   const long long N = (long long)a.N1 * N2;
-  V *row = reinterpret_cast<V *>(DIRECT ? a.d_state : I->state) + f * N + (long long)N2 * k1;
+  const auto row = gp(reinterpret_cast<V *>(DIRECT ? a.d_state : I->state)) + f * N + (long long)N2 * k1;
 
   constexpr int NTW = tw_count(N2, E, tw_pow<V>());
   V v[E], ph_tab[E], treg[NTW > 0 ? NTW : 1];
-  const V *fac = reinterpret_cast<const V *>(DIRECT ? a.d_lfac : I->lfac);
+  const auto fac = gp(reinterpret_cast<const V *>(DIRECT ? a.d_lfac : I->lfac));
   const bool use_fac = !a.use_gdd && fac;
-  const V *tab = a.use_gdd ? reinterpret_cast<const V *>(DIRECT ? a.d_gdd : I->gdd) + (f - 1) * N
-                          : reinterpret_cast<const V *>(DIRECT ? a.d_lin : I->lin) + f * N;
+  auto tab = a.use_gdd ? gp(reinterpret_cast<const V *>(DIRECT ? a.d_gdd : I->gdd)) + (f - 1) * N
+                     : gp(reinterpret_cast<const V *>(DIRECT ? a.d_lin : I->lin)) + f * N;
   // phase loads first (the factorised products only wait for these), then
   // the row, then the twiddles
   V xr = mk<V>(0, 0);
@@ -440,22 +440,22 @@ __global__ __launch_bounds__(row_max_threads(N2, E)) void row_stage(const OpoIns
     // factorised phase Z[E k1 + m] S[t + m T2] X(k1 t), X = Xhi Xlo
     const int L = a.xl_bits;
     const long long FB = (long long)E * a.N1 + N2 + ((N / E) >> L) + (1LL << L);
-    const V *F = fac + f * FB;
-    const V *S = F + E * a.N1;
-    const V *XH = S + N2;
-    const V *XL = XH + ((N / E) >> L);
+    const auto F = fac + f * FB;
+    const auto S = F + E * a.N1;
+    const auto XH = S + N2;
+    const auto XL = XH + ((N / E) >> L);
     const unsigned e = (unsigned)k1 * (unsigned)t;
-    xr = cmul(XH[e >> L], XL[e & ((1u << L) - 1u)]);
+    xr = cmul(V(XH[e >> L]), V(XL[e & ((1u << L) - 1u)]));
 #pragma unroll
-    for (int m = 0; m < E; ++m) ph_tab[m] = cmul(F[E * k1 + m], S[t + m * T2]);
+    for (int m = 0; m < E; ++m) ph_tab[m] = cmul(V(F[E * k1 + m]), V(S[t + m * T2]));
   } else {
-    tab += (long long)N2 * k1;
+    tab = tab + (long long)N2 * k1;
 #pragma unroll
     for (int m = 0; m < E; ++m) ph_tab[m] = tab[t + m * T2];
   }
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = row[t + m * T2];
-  load_twiddles<N2, E>(treg, t, reinterpret_cast<const V *>(a.tw2));
+  load_twiddles<N2, E>(treg, t, gp(reinterpret_cast<const V *>(a.tw2)));
   if (use_fac) {
 #pragma unroll
     for (int m = 0; m < E; ++m) ph_tab[m] = cmul(ph_tab[m], xr);
