@@ -302,7 +302,10 @@ def test_compaction_leaves_results_unchanged(gpu, monkeypatch):
     (OPO_COMPACT, on by default); instances that finished were skipped by
     every block before, so round trips, states, powers, power histories and
     fields equal those of a batch that keeps every instance in every launch,
-    bit for bit.  A second run on the same batch starts from all instances."""
+    bit for bit.  A second run on the same batch starts from all instances.
+    Rows of 256 threads make the compacted launches (at most 64 instances)
+    regroup the rows into one-wavefront blocks; the tuned row tile is back
+    after the run."""
     P = gpu
     cfgs = []
     for N in (1024, 2048):
@@ -316,6 +319,8 @@ def test_compaction_leaves_results_unchanged(gpu, monkeypatch):
         b = P.Batch([m[0] for m in made])
         for k, (_, x) in enumerate(made):
             b.upload(k, x["w"], x["T"], x["Ap_in"], x["Ap"], x["As"], x["Ai"])
+        for N in (1024, 2048):
+            b.set_tuning(N, row_threads=256)
         return b
 
     def result(b):
@@ -332,7 +337,11 @@ def test_compaction_leaves_results_unchanged(gpu, monkeypatch):
                 assert (a is None and c is None) or np.array_equal(a, c)
 
     b = fresh()
+    tuned = b.get_tuning()
+    # rows of N2 = N / 32 points at these sizes: threads per row block = row_tile * N2 / row_spt
+    assert all(t["row_tile"] * (t["N"] // 32) // t["row_spt"] > 64 for t in tuned)
     on = result(b)
+    assert b.get_tuning() == tuned
     for k, (_, x) in enumerate(made):  # the same batch again, from the same inputs
         b.upload(k, x["w"], x["T"], x["Ap_in"], x["Ap"], x["As"], x["Ai"])
     again = result(b)
