@@ -378,7 +378,10 @@ def headline_sweep(weights, count=256, subset=12, seed=31):
     warm_rate = float(steady.size / arm_s)
     cold_rate = cold_steady / cold_s
     sub_warm_steady = int(sum(1 for k in idx if st[k] == 1))
-    bz = 2 * 3 * N * 16 * count  # one read + one write of every field of the batch: bytes of one stage launch
+    # one read + one write of every field of the instances one stage launch covers (a batch of one grid
+    # size runs as two groups of half the points, each launched on its own stream; time_stages times the first)
+    launch_n = int(tuning[0]["count"])
+    bz = 2 * 3 * N * 16 * launch_n
     return {
         "value": warm_rate,
         "per_instance_fp64_cold": {
@@ -402,7 +405,7 @@ def headline_sweep(weights, count=256, subset=12, seed=31):
         "surrogate": {k: meta[k] for k in ("family", "points", "N", "NZ", "steady_states", "epochs", "val")
                       if k in meta},
         "cold_subset": cold,
-        "kernel": {"col_us": col_us, "row_us": row_us, "bz": bz,
+        "kernel": {"col_us": col_us, "row_us": row_us, "bz": bz, "launch_instances": launch_n,
                    "col_frac": bz / (col_us * 1e-6) / HBM_PEAK, "row_frac": bz / (row_us * 1e-6) / HBM_PEAK},
     }
 
@@ -426,7 +429,7 @@ def sweep_kernels(weights, count=256, rts=10, seed=31):
     wall = time.perf_counter() - t0
     col_us, row_us = b.time_stages(N, 20)
     b.close()
-    bz = 2 * 3 * N * 16 * count
+    bz = 2 * 3 * N * 16 * int(tuning[0]["count"])  # the first group's launch (see headline_sweep)
     return {"workload": f"{count}-point sPPLT sweep batch, N=2^14, 100 slices, FP64", "round_trips": rts,
             "round_trips_per_s_per_instance": rts / wall, "col_stage_us": col_us, "row_stage_us": row_us,
             "col_frac": bz / (col_us * 1e-6) / HBM_PEAK, "row_frac": bz / (row_us * 1e-6) / HBM_PEAK,
@@ -544,7 +547,8 @@ def main():
                    "per_instance_subset": a.subset},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": load_traffic(f"sweep_fp64_{dom}"),
-                     "kernel": dom, "kernel_us": dom_us, "algorithmic_bytes_per_launch": k["bz"]},
+                     "kernel": dom, "kernel_us": dom_us, "algorithmic_bytes_per_launch": k["bz"],
+                     "launch_instances": k["launch_instances"]},
         "extra": {"sweep": h},
     }
     out["extra"]["c2_single_fp64"] = c2_section(a)

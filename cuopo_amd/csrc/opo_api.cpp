@@ -799,6 +799,29 @@ int engine_run(Engine &E, int rt0, int n_rt, int check_every, double tol) {
   return OPO_OK;
 }
 
+// A batch of one grid size with at least kSplitMin instances runs as two
+// groups of half the instances each, on two streams (forked and joined inside
+// every round trip, as groups of different sizes are).  The two chains then
+// overlap: once compaction has shrunk the launches to a few instances, every
+// stage is latency-bound and one chain's launches fill the other's gaps
+// (headline sweep batch: 52.4 -> 46.9 s, results bit-identical; four groups
+// took 59.8 s).  Descriptors hold pointers, so the split changes no arithmetic.
+constexpr int kSplitMin = 64;
+
+void split_large_group(Engine &E) {
+  if (E.groups.size() != 1 || (int)E.groups[0].members.size() < kSplitMin) return;
+  const std::vector<int> all = E.groups[0].members;
+  const int N = E.groups[0].N, n = (int)all.size(), h = n / 2;
+  E.groups.clear();
+  E.groups.resize(2);
+  for (int q = 0; q < 2; ++q) {
+    Group &g = E.groups[q];
+    g.N = N;
+    g.members.assign(all.begin() + (q ? h : 0), q ? all.end() : all.begin() + h);
+    for (size_t j = 0; j < g.members.size(); ++j) E.where[g.members[j]] = {q, (int)j};
+  }
+}
+
 int engine_init(Engine &E, const opo_params *params, int count) {
   if (count < 1) return fail(OPO_ERR_INVALID, "count must be >= 1");
   for (int k = 0; k < count; ++k) RCHK(validate(&params[k]));
@@ -814,6 +837,9 @@ int engine_init(Engine &E, const opo_params *params, int count) {
   if (const char *ug = std::getenv("OPO_USE_GRAPH")) E.use_graph = ug[0] != '0';
   // OPO_COMPACT=0 keeps every instance in every launch of a converging batch run
   if (const char *cp = std::getenv("OPO_COMPACT")) E.compact = cp[0] != '0';
+  // OPO_SPLIT=0 keeps a large one-grid batch in one group (see split_large_group)
+  bool split = true;
+  if (const char *sp = std::getenv("OPO_SPLIT")) split = sp[0] != '0';
   E.count = count; E.NZ = p0.NZ; E.three_eqs = p0.three_eqs ? 1 : 0; E.regime = p0.regime;
   E.fp32 = p0.precision == OPO_FP32C; E.device = p0.device;
   E.NRT = 1;
@@ -835,6 +861,7 @@ int engine_init(Engine &E, const opo_params *params, int count) {
     E.where[k] = {gi, (int)E.groups[gi].members.size()};
     E.groups[gi].members.push_back(k);
   }
+  if (split) split_large_group(E);
   HIPCHK(hipEventCreateWithFlags(&E.fork_ev, hipEventDisableTiming));
   for (auto &g : E.groups) {
     for (int k : g.members) if (params[k].GDD != 0) g.any_gdd = true;
@@ -1459,9 +1486,22 @@ int set_tuning(Engine &E, Group &g, const opo_tuning *t) {
 // The timed slices mutate the fields, so each group's state is saved before
 // and restored after.
 int engine_autotune(Engine &E, int iters) {
-  for (auto &g : E.groups) {
+  for (size_t gi = 0; gi < E.groups.size(); ++gi) {
+    Group &g = E.groups[gi];
     for (auto &I : g.inst)
       if (!I.uploaded) return fail(OPO_ERR_STATE, "autotune before upload");
+    // the groups of one grid size (a large one-size batch split in two) share
+    // the settings tuned on the first of them
+    bool shared = false;
+    for (size_t j = 0; j < gi && !shared; ++j) {
+      if (E.groups[j].N != g.N) continue;
+      opo_tuning t;
+      get_tuning(E, E.groups[j], &t);
+      RCHK(set_tuning(E, g, &t));
+      g.tuned_us = E.groups[j].tuned_us;
+      shared = true;
+    }
+    if (shared) continue;
     const size_t bytes = (size_t)g.NF * g.N * esize(g);
     const int n = (int)g.members.size();
     void *bak = nullptr;
@@ -1548,12 +1588,6 @@ int engine_autotune(Engine &E, int iters) {
   return reset_counters(E, 0);
 }
 
-Group *group_of(Engine &E, int N) {
-  for (auto &g : E.groups)
-    if (g.N == N) return &g;
-  return nullptr;
-}
-
 }  // namespace
 
 int opo_autotune(opo_solver *s, int iters) {
@@ -1588,9 +1622,11 @@ int opo_batch_get_tuning(opo_batch *b, opo_tuning *out, int max) {
 
 int opo_batch_set_tuning(opo_batch *b, const opo_tuning *t) {
   if (!b || !t) return fail(OPO_ERR_INVALID, "null");
-  Group *g = group_of(b->E, t->N);
-  if (!g) return fail(OPO_ERR_INVALID, "no group with that N");
-  return set_tuning(b->E, *g, t);
+  int found = 0;  // every group of that grid size (a large batch of one size runs as two)
+  for (auto &g : b->E.groups)
+    if (g.N == t->N) { ++found; RCHK(set_tuning(b->E, g, t)); }
+  if (!found) return fail(OPO_ERR_INVALID, "no group with that N");
+  return OPO_OK;
 }
 
 int opo_batch_time_stages(opo_batch *b, int N, int iters, double *col_us, double *row_us) {

@@ -251,13 +251,17 @@ int opo_batch_power_history(opo_batch *b, int idx, double *out, int n);
 int opo_batch_power(opo_batch *b, double *out);
 int opo_batch_last_run_ms(opo_batch *b, double *ms);
 /* Per-group launch settings (autotuner; no reference counterpart).  The
- * instances of one grid size N form a group launched together. */
+ * instances of one grid size N form a group launched together; a batch of a
+ * single grid size with 64 or more instances runs as two groups of half the
+ * instances each, on two streams (OPO_SPLIT=0 in the environment at batch
+ * creation keeps one group). */
 int opo_batch_autotune(opo_batch *b, int iters);
 /* fills min(max, groups) entries, returns the number of groups */
 int opo_batch_get_tuning(opo_batch *b, opo_tuning *out, int max);
-/* applies t to the group with grid size t->N */
+/* applies t to every group with grid size t->N */
 int opo_batch_set_tuning(opo_batch *b, const opo_tuning *t);
-/* opo_time_stages for the group of grid size N. */
+/* opo_time_stages for the first group of grid size N (its count is in
+ * opo_batch_get_tuning). */
 int opo_batch_time_stages(opo_batch *b, int N, int iters, double *col_us, double *row_us);
 int opo_batch_set_tiles(opo_batch *b, int col_threads, int row_threads);
 /* Save slots for every instance (opo_enable_save per instance; the round

@@ -355,3 +355,40 @@ def test_compaction_leaves_results_unchanged(gpu, monkeypatch):
     assert min(r for r in rts if r > 0) < max(rts)
     same(on, off)
     same(on, again)
+
+
+def test_large_batch_runs_as_two_groups_with_unchanged_results(gpu, monkeypatch):
+    """A batch of one grid size with 64 or more instances runs as two groups
+    of half the instances on two streams (OPO_SPLIT=0 keeps one group); the
+    round trips, states, powers and fields are those of the one-group batch,
+    bit for bit, and a setting applied to the grid size reaches both groups."""
+    P = gpu
+    cfgs = []
+    for k in range(64):
+        c = c2(N=1024, NZ=4, NRT=1500)
+        c.Nth = [0.6, 1.5, 3.0, 8.0][k % 4] * (1.0 + 0.01 * k)
+        cfgs.append(c)
+    made = [instance(P, c, 300 + k, 1e-6) for k, c in enumerate(cfgs)]
+
+    def run():
+        b = P.Batch([m[0] for m in made])
+        for k, (_, x) in enumerate(made):
+            b.upload(k, x["w"], x["T"], x["Ap_in"], x["Ap"], x["As"], x["Ai"])
+        b.set_tuning(1024, col_threads=64, row_threads=128)
+        tun = b.get_tuning()
+        rts, st = b.run_ex(1500, check_every=50, tol=1e-3, off_ratio=1e-6)
+        out = (rts.tolist(), st.tolist(), b.power().tolist(), [b.download(k) for k in range(len(cfgs))], tun)
+        b.close()
+        return out
+
+    two = run()
+    monkeypatch.setenv("OPO_SPLIT", "0")  # read when the batch is created
+    one = run()
+    assert [t["count"] for t in two[4]] == [32, 32] and [t["count"] for t in one[4]] == [64]
+    assert all(t["row_tile"] == one[4][0]["row_tile"] and t["col_tile"] == one[4][0]["col_tile"] for t in two[4])
+    assert 1 in two[1] and 2 in two[1]  # instances finish at different round trips
+    assert two[:3] == one[:3]
+    for f1, f2 in zip(two[3], one[3]):
+        for a, c in zip(f1, f2):
+            assert (a is None and c is None) or np.array_equal(a, c)
+
