@@ -133,6 +133,7 @@ SIGNATURES = {
     "opo_batch_power": (_i, [_p, _p]),
     "opo_batch_power_history": (_i, [_p, _i, _p, _i]),
     "opo_batch_last_run_ms": (_i, [_p, C.POINTER(_d)]),
+    "opo_batch_set_accel": (_i, [_p, _i, _i]),
     "opo_batch_set_tiles": (_i, [_p, _i, _i]),
     "opo_batch_destroy": (_i, [_p]),
     "opo_mlp_create": (_i, [_i, _i, _i, _i, _i, _u64, _i, C.POINTER(_p)]),
@@ -484,6 +485,11 @@ class Batch:
 
     def set_tiles(self, col_threads=0, row_threads=0):
         _check(lib().opo_batch_set_tiles(self._h, col_threads, row_threads))
+
+    def set_accel(self, every=400, start=1000):
+        """Round-trip extrapolation of slowly converging instances in the
+        convergence-tested runs (every = 0 turns it off)."""
+        _check(lib().opo_batch_set_accel(self._h, every, start))
 
     def last_run_ms(self):
         v = C.c_double()

@@ -124,6 +124,22 @@ size_t lin_factor_len(int N, int N1, int xl_bits, int E);
 int lin_factors(const opo_params &p, const double *w, int N1, int xl_bits, int E, double *out);
 
 int launch_tick(OpoInst *insts, const TickArgs &a, void *stream);
+
+// Extrapolation of slowly converging round trips (batched runs).  For one
+// instance: the resonant fields (signal, and idler with three equations,
+// stored one after the other, time domain) as the convergence snapshot held
+// them at three checks M round trips apart (s0 oldest, s2 the latest), and the
+// instance's field state (pump, signal, idler at 0, N, 2N).
+struct AccelDesc {
+  const void *s0, *s1, *s2;
+  void *state;
+  double step;  // accel_apply: state <- s2 + step (s2 - R s1); 0 copies s2
+  double c, s;  // R = e^{+i theta} on the signal, e^{-i theta} on the idler
+};
+// Nine FP64 sums per instance and resonant field (out[(k * nres + f) * 9 + j]):
+// |s0|^2, |s1|^2, |s2|^2, then <s0,s1>, <s0,s2>, <s1,s2> (re, im), <a,b> = sum conj(a) b.
+int launch_accel_dots(int fp32, const AccelDesc *d, int count, int nres, long long N, double *out, void *stream);
+int launch_accel_apply(int fp32, const AccelDesc *d, int count, int nres, long long N, void *stream);
 // zero fill of `words` 8-byte words on `stream`
 int launch_zero(void *dst, long long words, void *stream);
 

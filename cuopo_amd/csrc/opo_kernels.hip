@@ -676,6 +676,81 @@ int launch_tick(OpoInst *insts, const TickArgs &a, void *stream) {
   return (int)hipGetLastError();
 }
 
+// ------------------------------------------------ round-trip extrapolation
+// accel_dots: one block per (instance, resonant field); each thread sums a
+// strided subset in order, then a fixed tree over the block (deterministic).
+template <typename T>
+__global__ __launch_bounds__(256) void accel_dots(const AccelDesc *__restrict__ d, int nres, long long N,
+                                                  double *__restrict__ out) {
+  using V = typename vec2<T>::type;
+  const AccelDesc D = d[blockIdx.x];
+  const long long off = (long long)blockIdx.y * N;
+  const auto a = gp(reinterpret_cast<const V *>(D.s0)) + off;
+  const auto b = gp(reinterpret_cast<const V *>(D.s1)) + off;
+  const auto c = gp(reinterpret_cast<const V *>(D.s2)) + off;
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (long long i = threadIdx.x; i < N; i += blockDim.x) {
+    const V x = a[i], y = b[i], z = c[i];
+    const double xr = x.x, xi = x.y, yr = y.x, yi = y.y, zr = z.x, zi = z.y;
+    acc[0] += xr * xr + xi * xi;
+    acc[1] += yr * yr + yi * yi;
+    acc[2] += zr * zr + zi * zi;
+    acc[3] += xr * yr + xi * yi; acc[4] += xr * yi - xi * yr;  // conj(x) y
+    acc[5] += xr * zr + xi * zi; acc[6] += xr * zi - xi * zr;  // conj(x) z
+    acc[7] += yr * zr + yi * zi; acc[8] += yr * zi - yi * zr;  // conj(y) z
+  }
+  __shared__ double red[9][256];
+#pragma unroll
+  for (int j = 0; j < 9; ++j) red[j][threadIdx.x] = acc[j];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+#pragma unroll
+      for (int j = 0; j < 9; ++j) red[j][threadIdx.x] += red[j][threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 9) out[((long long)blockIdx.x * nres + blockIdx.y) * 9 + threadIdx.x] = red[threadIdx.x][0];
+  (void)nres;
+}
+
+// accel_apply: the instance's resonant fields <- s2 + step (s2 - R s1)
+template <typename T>
+__global__ __launch_bounds__(256) void accel_apply(const AccelDesc *__restrict__ d, int nres, long long N) {
+  using V = typename vec2<T>::type;
+  const AccelDesc D = d[blockIdx.y];
+  const auto b = gp(reinterpret_cast<const V *>(D.s1));
+  const auto c = gp(reinterpret_cast<const V *>(D.s2));
+  const auto st = gp(reinterpret_cast<V *>(D.state)) + N;  // signal at N, idler at 2N
+  const long long n = (long long)nres * N;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const V y = b[i], z = c[i];
+    const double sg = i < N ? 1.0 : -1.0;  // idler rotates the other way
+    const double rr = (double)y.x * D.c - sg * (double)y.y * D.s, ri = (double)y.y * D.c + sg * (double)y.x * D.s;
+    V o;
+    o.x = (decltype(o.x))((double)z.x + D.step * ((double)z.x - rr));
+    o.y = (decltype(o.y))((double)z.y + D.step * ((double)z.y - ri));
+    st[i] = o;
+  }
+}
+
+int launch_accel_dots(int fp32, const AccelDesc *d, int count, int nres, long long N, double *out, void *stream) {
+  if (count <= 0) return 0;
+  dim3 grid((unsigned)count, (unsigned)nres);
+  if (fp32) hipLaunchKernelGGL(accel_dots<float>, grid, dim3(256), 0, (hipStream_t)stream, d, nres, N, out);
+  else hipLaunchKernelGGL(accel_dots<double>, grid, dim3(256), 0, (hipStream_t)stream, d, nres, N, out);
+  return (int)hipGetLastError();
+}
+
+int launch_accel_apply(int fp32, const AccelDesc *d, int count, int nres, long long N, void *stream) {
+  if (count <= 0) return 0;
+  const long long per = std::min<long long>(((long long)nres * N + 255) / 256, 64);
+  dim3 grid((unsigned)per, (unsigned)count);
+  if (fp32) hipLaunchKernelGGL(accel_apply<float>, grid, dim3(256), 0, (hipStream_t)stream, d, nres, N);
+  else hipLaunchKernelGGL(accel_apply<double>, grid, dim3(256), 0, (hipStream_t)stream, d, nres, N);
+  return (int)hipGetLastError();
+}
+
 // zero fill of n 8-byte words (grid-stride; a kernel rather than a memset so
 // the clear is an ordinary launch on the caller's stream)
 __global__ __launch_bounds__(256) void zero_kernel(unsigned long long *__restrict__ dst, long long n) {

@@ -265,7 +265,7 @@ def family_batch(points, N, NZ, max_rt, precision=FP32C, seed=0, noise_sigma=1e-
 
 
 def run_family(points, N=4096, NZ=25, max_rt=20000, check_every=50, tol=1e-6, off_ratio=1e-6, precision=FP32C,
-               seed=0, noise_sigma=1e-20, warm=None, autotune=True, fields=True):
+               seed=0, noise_sigma=1e-20, warm=None, autotune=True, fields=True, accel=None):
     """Run family points in one batch to the convergence test.  Returns a dict:
     rts, states (1 steady, 2 off, 0 capped, -1 diverged), power, device ms,
     the derived scalars and (fields=True) the final (Ap, As, Ai) per point."""
@@ -273,6 +273,8 @@ def run_family(points, N=4096, NZ=25, max_rt=20000, check_every=50, tol=1e-6, of
     try:
         if autotune:
             b.autotune(iters=3)
+        if accel:
+            b.set_accel(*accel)  # (every, start): round-trip extrapolation, see Batch.set_accel
         rts, st = b.run_ex(max_rt, check_every=check_every, tol=tol, off_ratio=off_ratio)
         out = dict(rts=rts, states=st, power=b.power(), device_ms=b.last_run_ms(), derived=ds)
         if fields:

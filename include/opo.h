@@ -250,6 +250,18 @@ int opo_batch_power_history(opo_batch *b, int idx, double *out, int n);
 /* final sum |As|^2 per instance (count values) */
 int opo_batch_power(opo_batch *b, double *out);
 int opo_batch_last_run_ms(opo_batch *b, double *ms);
+/* Round-trip extrapolation for the convergence-tested runs (opo_batch_run,
+ * opo_batch_run_ex; no reference counterpart: the reference runs a fixed
+ * number of round trips).  every > 0: every `every` round trips (a multiple
+ * of check_every) from `start` on, an active instance whose last three
+ * snapshots show one geometric relaxation mode (ratio 0.05..0.995 per
+ * `every` round trips, the last step within 10% of the ratio times the one
+ * before) jumps to the series' limit and iterates on from there; its steady
+ * state is the same fixed point of the round trip, reached in fewer round
+ * trips.  every = 0 (the default) turns it off; results are then those of
+ * plain iteration.  Device memory: two copies of the resonant fields per
+ * instance that reaches `start`. */
+int opo_batch_set_accel(opo_batch *b, int every, int start);
 /* Per-group launch settings (autotuner; no reference counterpart).  The
  * instances of one grid size N form a group launched together; a batch of a
  * single grid size with 64 or more instances runs as two groups of half the
