@@ -615,6 +615,8 @@ int reset_counters(Engine &E, int rt0) {
   E.inst_dirty = true;
   RCHK(sync_insts(E));  // descriptors carry active_v = 1
   for (auto &g : E.groups) {
+    for (auto &I : g.inst) I.acc_n = 0;  // extrapolation histories belong to one run
+    g.act.clear();
     const int n = (int)g.members.size();
     std::vector<int> rt(n, rt0), act(n, 1), conv(n, -1), stat(n, 0);
     std::vector<double> pref(n, 0.0);
