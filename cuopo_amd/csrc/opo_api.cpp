@@ -593,6 +593,7 @@ int emit_start(Engine &E, Group &g) {
 // one round trip after emit_start (or after a previous non-final round trip)
 int emit_rt_body(Engine &E, Group &g, bool last, int check_every, double tol) {
   using namespace opo;
+  if (g.launch_n == 0) return OPO_OK;  // every instance of the group has finished (compaction)
   const int A = all_mask(g), SI = si_mask(g), NZ = E.NZ;
   for (int s = 0; s < NZ; ++s) {
     RCHK(emit_row(E, g, false, A));
@@ -693,9 +694,9 @@ int build_graph(Engine &E, int check_every, double tol) {
 // compaction gets a descriptor array of the active ones, and its launches
 // shrink to them (*changed; the caller captures the round trip again, a few
 // ms against the 50 round trips between checks); a group with none left
-// keeps one inactive descriptor, whose blocks return at once.  Finished
-// instances were skipped by every block anyway, so the arithmetic and the
-// results do not change.
+// launches nothing (its chain of a two-group batch no longer runs beside the
+// other's).  Finished instances were skipped by every block anyway, so the
+// arithmetic and the results do not change.
 constexpr int kNarrowRowsBelow = 64;  // launches of this many instances or fewer get one-wavefront row blocks
 
 int check_active(Engine &E, bool *alive, bool *changed) {
@@ -712,14 +713,14 @@ int check_active(Engine &E, bool *alive, bool *changed) {
       if (act[k]) h.push_back(make_desc(E, g, k));
     if (!h.empty()) *alive = true;
     const int m = (int)h.size();
-    if (!E.compact || g.launch_n <= 1 || m >= g.launch_n) continue;
-    if (m == 0) {
-      for (int k = 0; k < n && h.empty(); ++k) h.push_back(make_desc(E, g, k));
-      h[0].active_v = 0;
+    if (!E.compact || m >= g.launch_n) continue;
+    if (m > 0) {
+      HIPCHK(hipMemcpyAsync(g.d_insts, h.data(), h.size() * sizeof(OpoInst), hipMemcpyHostToDevice, E.stream));
+      HIPCHK(hipStreamSynchronize(E.stream));
     }
-    HIPCHK(hipMemcpyAsync(g.d_insts, h.data(), h.size() * sizeof(OpoInst), hipMemcpyHostToDevice, E.stream));
-    HIPCHK(hipStreamSynchronize(E.stream));
-    g.launch_n = (int)h.size();
+    g.launch_n = m;  // 0: the group is done and launches nothing until the next run
+    *changed = true;
+    if (m == 0) continue;
     // A few instances leave most CUs idle and every stage latency-bound; rows
     // of at most one wavefront per block then spread them over more CUs (the
     // tail of a sweep at 2^14 samples: 2-7% less time per round trip at 1-64
@@ -731,7 +732,6 @@ int check_active(Engine &E, bool *alive, bool *changed) {
       if (!g.rt_tuned) g.rt_tuned = g.rt;
       g.rt = one_wave_rt;
     }
-    *changed = true;
   }
   return OPO_OK;
 }
