@@ -9,7 +9,7 @@ start, the steady power against the cold start's, and the field error against
 the cold start's fields (signal and idler aligned by the free gauge phase),
 next to the cold-vs-cold spread of the same numbers.
 
-    python -m cuopo_amd.ab_surrogate --out gpurun_out/ab.json
+    python -m cuopo_amd.ab_surrogate --out ab.json
 """
 import argparse
 import json

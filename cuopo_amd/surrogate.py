@@ -28,7 +28,7 @@ converges in a few hundred round trips where the cold start from noise needs
 thousands.  The warm start is therefore the predicted spectrum's dominant
 mode pair at the predicted power (random gauge phase); the solver iterates it
 to its own convergence test, and the cold start stays the fallback and the
-judge of the warm state's power and field.
+reference for the warm state's power and field.
 
 Data flow: `generate_dataset` runs batched sweeps (opo_batch_*) from seeded
 noise to the convergence test and keeps the points that reached a steady
