@@ -158,7 +158,7 @@ constexpr int tw_off(int M, int Q, int E, bool pow) {
                       (pow ? 1 : radix_at(M, Q - 1, E) - 1) * (E / radix_at(M, Q - 1, E));
 }
 constexpr int tw_count(int M, int E, bool pow) { return tw_off(M, npasses(M, E), E, pow); }
-template <typename V> constexpr bool tw_pow() { return sizeof(V) == 16; }  // double2
+template <typename V> constexpr bool tw_pow() { return std::is_same<V, double2>::value; }
 
 // u[j] *= w^j for j = 1 .. R-1, the powers by squaring
 template <int R, typename V> __device__ __forceinline__ void mul_root_powers(V *u, V w) {
