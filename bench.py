@@ -8,8 +8,8 @@ complex128 samples per round trip, 100 z-slices per pass) runs as one FP64
 batch from the learned warm start (the MLP's predicted mode pair and power)
 to the steady-state test: intracavity power change below 1e-6 over 50 round
 trips, cap 20,000 round trips (a cold start from 1e-20 noise needs ~10^4),
-with the batch's round-trip extrapolation on (every 400 round trips from
-1000 on, a point relaxing geometrically towards its steady state jumps to the
+with the batch's round-trip extrapolation on (every 200 round trips from
+400 on, a point relaxing geometrically towards its steady state jumps to the
 limit).  Counted: points that pass that test.  Beside it,
 the plain per-instance FP64 solver cold-starts from 1e-20 noise on a subset
 of the same points, one point after another, to the same test or cap, and
@@ -331,7 +331,7 @@ def c3_sweep_cold():
 
 
 FAMILY_N, FAMILY_NZ, SWEEP_CAP = 1 << 14, 100, 20000
-SWEEP_ACCEL = (400, 1000)  # extrapolation every 400 round trips from 1000 on (opo_batch_set_accel)
+SWEEP_ACCEL = (200, 400)  # extrapolation every 200 round trips from 400 on (opo_batch_set_accel)
 
 
 def headline_sweep(weights, count=256, subset=12, seed=31):
@@ -548,8 +548,8 @@ def main():
         "config": {"workload": "256-point CW three-equation sPPLT sweep (532 -> 1000 + 1137 nm; pump 1.2-5x "
                                "threshold, detuning -pi..pi rad, crystal 5-40 mm, GDD -500..500 fs^2, R 0.90-0.99), "
                                "N=2^14 complex128 samples, 100 z-slices, one batch from the learned warm start to "
-                               "the steady test (|dP| < 1e-6 P over 50 round trips, round-trip extrapolation every 400 "
-                               "round trips from 1000 on), cap 20000 round trips",
+                               "the steady test (|dP| < 1e-6 P over 50 round trips, round-trip extrapolation every "
+                               f"{SWEEP_ACCEL[0]} round trips from {SWEEP_ACCEL[1]} on), cap {SWEEP_CAP} round trips",
                    "points": 256, "N": FAMILY_N, "NZ": FAMILY_NZ, "cap": SWEEP_CAP,
                    "per_instance_subset": a.subset},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
