@@ -8,8 +8,8 @@ complex128 samples per round trip, 100 z-slices per pass) runs as one FP64
 batch from the learned warm start (the MLP's predicted mode pair and power)
 to the steady-state test: intracavity power change below 1e-6 over 50 round
 trips, cap 20,000 round trips (a cold start from 1e-20 noise needs ~10^4),
-with the batch's round-trip extrapolation on (every 200 round trips from
-400 on, a point relaxing geometrically towards its steady state jumps to the
+with the batch's round-trip extrapolation on (every 150 round trips from
+300 on, a point relaxing geometrically towards its steady state jumps to the
 limit).  Counted: points that pass that test.  Beside it,
 the plain per-instance FP64 solver cold-starts from 1e-20 noise on a subset
 of the same points, one point after another, to the same test or cap, and
@@ -331,7 +331,7 @@ def c3_sweep_cold():
 
 
 FAMILY_N, FAMILY_NZ, SWEEP_CAP = 1 << 14, 100, 20000
-SWEEP_ACCEL = (200, 400)  # extrapolation every 200 round trips from 400 on (opo_batch_set_accel)
+SWEEP_ACCEL = (150, 300)  # extrapolation every 150 round trips from 300 on (opo_batch_set_accel)
 
 
 def headline_sweep(weights, count=256, subset=12, seed=31):
