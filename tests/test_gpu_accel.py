@@ -57,13 +57,14 @@ def test_extrapolation_reaches_the_same_steady_states_in_fewer_round_trips(gpu):
 
 
 @pytest.mark.parametrize("fp32c", [False, True])
-def test_extrapolation_on_small_three_wave_batches(gpu, fp32c):
+def test_extrapolation_on_small_three_wave_batches(gpu, fp32c, monkeypatch):
     """24 small three-equation CW points (1.1-3x threshold, detuning -1..1)
     from 1e-6 noise, FP64 and FP32C: extrapolation saves round trips; a
     point's outcome may only move from "not converged" to a final state
     (a below-threshold point decaying at 3e-4 per round trip extrapolates
     to zero and stops as "off" long before plain iteration would); points
-    steady in both agree in power to 1e-4, most of them bit for bit."""
+    steady in both agree in power to 1e-4, most of them bit for bit.  With
+    compaction off, the extrapolated run is the same bit for bit."""
     P = gpu
     from oracle import oracle as O
     from tests.helpers import c2, to_args
@@ -93,6 +94,9 @@ def test_extrapolation_on_small_three_wave_batches(gpu, fp32c):
 
     r0, s0, p0 = run(False)
     r1, s1, p1 = run(True)
+    monkeypatch.setenv("OPO_COMPACT", "0")  # finished instances stay in every launch, and in every restart
+    r2, s2, p2 = run(True)
+    assert np.array_equal(r1, r2) and np.array_equal(s1, s2) and np.array_equal(p1, p2)
     assert r1.sum() < r0.sum()
     moved = s0 != s1
     assert np.all(s0[moved] == 0) and np.all(s1[moved] != -1)
