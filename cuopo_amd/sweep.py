@@ -66,12 +66,16 @@ def make_batch(points, NZ=100, max_rt=5000, precision=FP32C, seed=1000, sigma=1e
 
 
 def run_batched(points, NZ=100, max_rt=5000, check_every=50, tol=1e-6, off_ratio=1e-6, precision=FP32C,
-                seed=1000, sigma=1e-20, warm=None, tiles=(0, 0), autotune=True):
+                seed=1000, sigma=1e-20, warm=None, tiles=(0, 0), autotune=True, accel=None):
     """All points in one batch.  Returns a dict with per-point round trips,
     states (1 steady, 2 off, 0 capped, -1 diverged), final powers, the
-    device time of the run and the wall time including uploads."""
+    device time of the run and the wall time including uploads.  accel =
+    (every, start) turns on the batch's round-trip extrapolation
+    (Batch.set_accel)."""
     t0 = time.perf_counter()
     b, made = make_batch(points, NZ, max_rt, precision, seed, sigma, warm, tiles, autotune)
+    if accel:
+        b.set_accel(*accel)
     t1 = time.perf_counter()
     rts, st = b.run_ex(max_rt, check_every=check_every, tol=tol, off_ratio=off_ratio)
     t2 = time.perf_counter()
