@@ -211,6 +211,25 @@ def test_diverged_warm_start_falls_back_to_the_cold_start(gpu):
             r["fallback_batch"].close()
 
 
+def test_sweep_driver_passes_the_extrapolation_schedule_on(gpu):
+    """run_batched(accel=(every, start)) runs exactly as a batch given the
+    same schedule through Batch.set_accel (bit for bit)."""
+    P = gpu
+    from cuopo_amd import sweep as SW
+    pts = [(1024, 3.0, 0.0, 20.0, 0.0, 0.95), (1024, 1.4, 0.3, 20.0, 0.0, 0.95), (2048, 3.0, -0.2, 20.0, 0.0, 0.95)]
+    kw = dict(NZ=4, max_rt=1500, precision=P.FP64, sigma=1e-3)
+    r = SW.run_batched(pts, autotune=False, accel=(100, 200), **kw)
+    b, _ = SW.make_batch(pts, **kw)
+    try:
+        b.set_accel(100, 200)
+        rts, st = b.run_ex(1500, check_every=50, tol=1e-6, off_ratio=1e-6)
+        assert np.array_equal(r["rts"], rts) and np.array_equal(r["states"], st)
+        assert np.array_equal(r["power"], b.power())
+    finally:
+        b.close()
+        r["batch"].close()
+
+
 def test_graph_cache_follows_the_convergence_settings(gpu):
     """The captured round-trip graph is reused only for the same convergence
     settings: a batch run with tol = 0.02, then re-run with tol = 1e-6, matches
