@@ -1390,6 +1390,49 @@ int engine_time_slices(Engine &E, Group &g, int iters, double *slice_us) {
   return OPO_OK;
 }
 
+// Every group's slices side by side, as a round trip runs them: between one
+// fork and one join each group's stream runs `iters` back-to-back (column,
+// row) pairs; *slice_us is the time from the fork to the join per slice.
+int engine_time_joint(Engine &E, int iters, double *slice_us) {
+  using namespace opo;
+  RCHK(sync_insts(E));
+  RCHK(reset_counters(E, 0));
+  auto slices = [&](int k) -> int {
+    RCHK(fork_groups(E));
+    int r = OPO_OK;
+    for (auto &g : E.groups) {
+      const int A = all_mask(g);
+      for (int it = 0; it < k && r == OPO_OK; ++it) {
+        r = emit_col(E, g, OP_NL_END | OP_NL_START, A, A, A, A, 0, 1 % E.NZ, 0, 0);
+        if (r == OPO_OK) r = emit_row(E, g, false, A);
+      }
+    }
+    const int j = join_groups(E);
+    return r != OPO_OK ? r : j;
+  };
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int rc = OPO_OK;
+  for (auto &e : ev)
+    if (rc == OPO_OK && hipEventCreate(&e) != hipSuccess) rc = fail(OPO_ERR_HIP, "event create");
+  if (rc == OPO_OK) rc = slices(1);
+  if (rc == OPO_OK && hipEventRecord(ev[0], E.stream) != hipSuccess) rc = fail(OPO_ERR_HIP, "event record");
+  if (rc == OPO_OK) rc = slices(iters);
+  if (rc == OPO_OK && hipEventRecord(ev[1], E.stream) != hipSuccess) rc = fail(OPO_ERR_HIP, "event record");
+  float ms = 0;
+  if (rc == OPO_OK && (hipEventSynchronize(ev[1]) != hipSuccess || hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess))
+    rc = fail(OPO_ERR_HIP, "event");
+  if (rc != OPO_OK) {
+    for (auto &g : E.groups)
+      if (g.gs) (void)hipStreamSynchronize(g.gs);
+    (void)hipStreamSynchronize(E.stream);
+  }
+  for (auto &e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (rc != OPO_OK) return rc;
+  *slice_us = 1e3 * ms / iters;
+  return OPO_OK;
+}
+
 int opo_time_stages(opo_solver *s, int iters, double *col_us, double *row_us) {
   if (!s || iters < 1) return fail(OPO_ERR_INVALID, "bad arguments");
   HIPCHK(hipSetDevice(s->E.device));
@@ -1615,6 +1658,108 @@ int set_tuning(Engine &E, Group &g, const opo_tuning *t) {
   return OPO_OK;
 }
 
+bool same_setting(const opo_tuning &a, const opo_tuning &b) {
+  return a.col_threads == b.col_threads && a.row_threads == b.row_threads && a.xcd_pack == b.xcd_pack &&
+         a.phase_mode == b.phase_mode && a.col_spt == b.col_spt && a.row_spt == b.row_spt;
+}
+
+// A round trip runs the groups of different grid sizes side by side on their
+// streams, and a setting timed for its group alone need not be the best beside
+// the others (on the 256-point batch of four grid sizes, the N = 2^15 group's
+// pick on every group ran the round trip 2-6% faster than each group's own
+// pick).  So each group's pick is also tried on every group, with all groups
+// timed together (engine_time_joint, the better of two timings), and the
+// fastest assignment is kept.  The fields are saved before and restored after.
+int autotune_joint(Engine &E, int iters) {
+  const size_t G = E.groups.size();
+  std::vector<opo_tuning> own(G);
+  std::vector<double> own_us(G);
+  for (size_t i = 0; i < G; ++i) {
+    get_tuning(E, E.groups[i], &own[i]);
+    own_us[i] = E.groups[i].tuned_us;
+  }
+  std::vector<std::vector<opo_tuning>> cands;
+  for (size_t j = 0; j < G; ++j) {
+    std::vector<opo_tuning> c(G);
+    bool differs = false;
+    for (size_t i = 0; i < G; ++i) {
+      c[i] = own[j];
+      c[i].N = own[i].N;
+      c[i].count = own[i].count;
+      differs = differs || !same_setting(c[i], own[i]);
+    }
+    bool dup = false;
+    for (const auto &x : cands) {
+      bool eq = true;
+      for (size_t i = 0; i < G; ++i) eq = eq && same_setting(x[i], c[i]);
+      dup = dup || eq;
+    }
+    if (differs && !dup) cands.push_back(c);
+  }
+  if (cands.empty()) return OPO_OK;  // one grid size, or every group took the same setting
+
+  std::vector<void *> bak(G, nullptr);
+  struct Frees {
+    std::vector<void *> &v;
+    ~Frees() {
+      for (void *p : v)
+        if (p) (void)hipFree(p);
+    }
+  } frees{bak};
+  for (size_t i = 0; i < G; ++i) {
+    Group &g = E.groups[i];
+    const size_t bytes = (size_t)g.NF * g.N * esize(g);
+    HIPCHK(hipMalloc(&bak[i], bytes * g.members.size()));
+    for (size_t k = 0; k < g.members.size(); ++k)
+      HIPCHK(hipMemcpyAsync((char *)bak[i] + bytes * k, g.inst[k].state, bytes, hipMemcpyDeviceToDevice, E.stream));
+  }
+  HIPCHK(hipStreamSynchronize(E.stream));
+  auto apply = [&](const std::vector<opo_tuning> &ts) -> bool {
+    for (size_t i = 0; i < G; ++i) {
+      opo_tuning t = ts[i];
+      if (set_tuning(E, E.groups[i], &t) != OPO_OK) return false;
+    }
+    return true;
+  };
+  auto timed = [&](double *us) -> int {
+    double a = 0, b = 0;
+    RCHK(engine_time_joint(E, iters, &a));
+    RCHK(engine_time_joint(E, iters, &b));
+    *us = std::min(a, b);
+    return OPO_OK;
+  };
+  double best_us = 0;
+  int rc = timed(&best_us);  // every group's own pick is in place
+  int best = -1;             // -1: the own picks
+  for (size_t c = 0; c < cands.size() && rc == OPO_OK; ++c) {
+    if (!apply(cands[c])) continue;  // the setting does not fit some group's kernels
+    double us = 0;
+    rc = timed(&us);
+    if (rc == OPO_OK && us < best_us) { best_us = us; best = (int)c; }
+  }
+  // the winner in place, with each group's own slice time under it
+  if (!apply(best < 0 ? own : cands[(size_t)best]))
+    return fail(OPO_ERR_INVALID, "internal: a tuned setting no longer applies");
+  for (size_t i = 0; i < G; ++i) {
+    Group &g = E.groups[i];
+    if (best < 0 || same_setting(cands[(size_t)best][i], own[i])) {
+      g.tuned_us = own_us[i];
+    } else if (rc == OPO_OK) {
+      double u = 0;
+      rc = engine_time_slices(E, g, iters, &u);
+      g.tuned_us = u;
+    }
+  }
+  for (size_t i = 0; i < G; ++i) {
+    Group &g = E.groups[i];
+    const size_t bytes = (size_t)g.NF * g.N * esize(g);
+    for (size_t k = 0; k < g.members.size(); ++k)
+      HIPCHK(hipMemcpyAsync(g.inst[k].state, (char *)bak[i] + bytes * k, bytes, hipMemcpyDeviceToDevice, E.stream));
+  }
+  HIPCHK(hipStreamSynchronize(E.stream));
+  return rc;
+}
+
 // Time the candidates of each group with engine_time_slices (back-to-back
 // slices, as a round trip runs them; the better of two timings) and keep the
 // fastest: for each XCD packing, the column settings (tile of 1/2, 1 or 2
@@ -1622,6 +1767,7 @@ int set_tuning(Engine &E, Group &g, const opo_tuning *t) {
 // row settings fixed, then the row settings (64..512-thread tiles, and 32 for
 // small groups, x 4 or 8 samples per thread x phase source) with the best
 // column settings, then the column settings once more beside those rows.
+// With several grid sizes, autotune_joint then weighs the picks side by side.
 // The timed slices mutate the fields, so each group's state is saved before
 // and restored after.
 int engine_autotune(Engine &E, int iters) {
@@ -1723,6 +1869,7 @@ int engine_autotune(Engine &E, int iters) {
       HIPCHK(hipMemcpyAsync(g.inst[k].state, (char *)bak + bytes * k, bytes, hipMemcpyDeviceToDevice, E.stream));
     HIPCHK(hipStreamSynchronize(E.stream));
   }
+  RCHK(autotune_joint(E, iters));
   RCHK(sync_insts(E));
   return reset_counters(E, 0);
 }

@@ -266,7 +266,10 @@ int opo_batch_set_accel(opo_batch *b, int every, int start);
  * instances of one grid size N form a group launched together; a batch of a
  * single grid size with 64 or more instances runs as two groups of half the
  * instances each, on two streams (OPO_SPLIT=0 in the environment at batch
- * creation keeps one group). */
+ * creation keeps one group).  opo_batch_autotune tunes each group as
+ * opo_autotune does; with several grid sizes it then also times every
+ * group's pick applied to all groups, all groups running side by side as in
+ * a round trip, and keeps the fastest assignment. */
 int opo_batch_autotune(opo_batch *b, int iters);
 /* fills min(max, groups) entries, returns the number of groups */
 int opo_batch_get_tuning(opo_batch *b, opo_tuning *out, int max);
