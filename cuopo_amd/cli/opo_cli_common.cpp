@@ -3,7 +3,10 @@
 #include "opo_cli_common.h"
 
 #include <algorithm>
+#include <cmath>
+#include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <ctime>
 #include <future>
 #include <string>
@@ -24,7 +27,104 @@ bool write_bin(const std::string &path, const double *v, size_t n) {
 }
 
 bool write_dat(const std::string &path, const double *v, size_t n, int threads);
+
+using u128 = unsigned __int128;
+
+// 5^q for q = 0..32: 5^32 < 2^75, so m 5^q < 2^128 for a 53-bit significand m
+struct Pow5 {
+  u128 p[33];
+  Pow5() {
+    p[0] = 1;
+    for (int q = 1; q <= 32; ++q) p[q] = p[q - 1] * 5;
+  }
+};
+const Pow5 kPow5;
+const u128 kTen19 = (u128)10000000000000000000ULL, kTen20 = kTen19 * 10;
+
+// t = floor(m 2^e 10^q) and whether the exact rest rounds it up (to nearest,
+// ties to even); false when that does not fit the 128-bit arithmetic
+bool scale10(uint64_t m, int e, int q, u128 *t, bool *up) {
+  if (q < 0 || q > 32) return false;
+  const u128 n = (u128)m * kPow5.p[q];
+  const int sh = e + q;  // m 2^e 10^q = n 2^sh
+  if (sh >= 0) {
+    if (sh >= 64 || (n >> (127 - sh)) != 0) return false;
+    *t = n << sh;
+    *up = false;
+    return true;
+  }
+  const int s = -sh;
+  if (s >= 127) return false;
+  const u128 r = n & (((u128)1 << s) - 1), half = (u128)1 << (s - 1);
+  *t = n >> s;
+  *up = r > half || (r == half && (*t & 1));
+  return true;
+}
+
+// the 20 decimal digits of d (10^19 <= d < 10^20)
+void digits20(u128 d, char *out) {
+  uint64_t hi = (uint64_t)(d / 10000000000ULL), lo = (uint64_t)(d % 10000000000ULL);
+  for (int i = 9; i >= 0; --i) { out[10 + i] = (char)('0' + lo % 10); lo /= 10; }
+  for (int i = 9; i >= 0; --i) { out[i] = (char)('0' + hi % 10); hi /= 10; }
+}
 }  // namespace
+
+int format_g20(double v, char *buf) {
+  uint64_t bits;
+  std::memcpy(&bits, &v, sizeof bits);
+  const int E = (int)((bits >> 52) & 0x7ff);
+  if (E == 0 || E == 0x7ff) return std::snprintf(buf, 32, "%.20g", v);  // zero, subnormal, inf, nan
+  const uint64_t m = (bits & ((1ULL << 52) - 1)) | (1ULL << 52);
+  const int e = E - 1075;  // |v| = m 2^e, 2^(e+52) <= |v| < 2^(e+53)
+  // X = floor(log10 |v|): an estimate from the binary exponent, corrected by
+  // the exact truncation (t must have exactly 20 digits)
+  int X = (int)std::floor((e + 52) * 0.30102999566398120);
+  u128 t = 0;
+  bool up = false, found = false;
+  for (int tries = 0; tries < 3 && !found; ++tries) {
+    if (!scale10(m, e, 19 - X, &t, &up)) return std::snprintf(buf, 32, "%.20g", v);
+    if (t < kTen19) --X;
+    else if (t >= kTen20) ++X;
+    else found = true;
+  }
+  if (!found) return std::snprintf(buf, 32, "%.20g", v);
+  u128 d = t + (up ? 1 : 0);
+  if (d == kTen20) { d = kTen19; ++X; }  // rounding carried into a new digit
+  char dg[20];
+  digits20(d, dg);
+  int nd = 20;  // significant digits after dropping trailing zeros (%g without '#')
+  while (nd > 1 && dg[nd - 1] == '0') --nd;
+  char *o = buf;
+  if (bits >> 63) *o++ = '-';
+  if (X >= -4 && X < 20) {  // style f with 19 - X decimals
+    if (X >= 0) {
+      for (int i = 0; i <= X; ++i) *o++ = dg[i];
+      if (nd > X + 1) {
+        *o++ = '.';
+        for (int i = X + 1; i < nd; ++i) *o++ = dg[i];
+      }
+    } else {
+      *o++ = '0';
+      *o++ = '.';
+      for (int i = 0; i < -X - 1; ++i) *o++ = '0';
+      for (int i = 0; i < nd; ++i) *o++ = dg[i];
+    }
+  } else {  // style e
+    *o++ = dg[0];
+    if (nd > 1) {
+      *o++ = '.';
+      for (int i = 1; i < nd; ++i) *o++ = dg[i];
+    }
+    *o++ = 'e';
+    *o++ = X < 0 ? '-' : '+';
+    const int ax = X < 0 ? -X : X;
+    if (ax >= 100) *o++ = (char)('0' + ax / 100);
+    *o++ = (char)('0' + (ax / 10) % 10);
+    *o++ = (char)('0' + ax % 10);
+  }
+  *o = 0;
+  return (int)(o - buf);
+}
 
 int fetch_and_write(int count, const std::function<int(int)> &fetch, const std::function<void(int)> &write) {
   std::vector<std::promise<int>> done(count > 0 ? count : 0);
@@ -72,26 +172,36 @@ bool write_dat(const std::string &path, const double *v, size_t n, int threads) 
   const size_t nblocks = (n + block - 1) / block;
   int nt = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
   nt = (int)std::min<size_t>((size_t)nt, std::max<size_t>(nblocks, 1));
-  bool ok = true;
+  bool ok = true, wrote = true;
+  std::vector<std::string> text, out;  // the batch being formatted, the batch being written
+  std::thread writer;                  // writes `out` while the next batch is formatted
   for (size_t b0 = 0; b0 < nblocks && ok; b0 += (size_t)nt) {
     const size_t nb = std::min<size_t>((size_t)nt, nblocks - b0);
-    std::vector<std::string> text(nb);
+    text.assign(nb, std::string());
     auto work = [&](size_t j) {
       const size_t lo = (b0 + j) * block, hi = std::min(n, lo + block);
       std::string &s = text[j];
       s.reserve((hi - lo) * 26);
       char buf[40];
       for (size_t i = lo; i < hi; ++i) {
-        const int len = std::snprintf(buf, sizeof buf, "%.20g\n", v[i]);
-        s.append(buf, (size_t)len);
+        const int len = format_g20(v[i], buf);
+        buf[len] = '\n';
+        s.append(buf, (size_t)len + 1);
       }
     };
     std::vector<std::thread> pool;
     for (size_t j = 1; j < nb; ++j) pool.emplace_back(work, j);
     work(0);
     for (auto &t : pool) t.join();
-    for (auto &s : text) ok = ok && std::fwrite(s.data(), 1, s.size(), f) == s.size();
+    if (writer.joinable()) writer.join();
+    ok = wrote;
+    out.swap(text);
+    writer = std::thread([&] {
+      for (const auto &s : out) wrote = wrote && std::fwrite(s.data(), 1, s.size(), f) == s.size();
+    });
   }
+  if (writer.joinable()) writer.join();
+  ok = ok && wrote;
   return std::fclose(f) == 0 && ok;
 }
 }  // namespace

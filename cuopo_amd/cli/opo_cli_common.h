@@ -19,8 +19,14 @@ namespace opo_cli {
 enum class Format { Dat, Bin, Both };
 bool set_format(const std::string &name);
 
+// printf("%.20g") of v into buf (at least 32 bytes, NUL-terminated); returns
+// the length.  Finite values from 1e-13 up to 1e20 in magnitude take an exact
+// integer path (the value's 20 leading decimal digits, rounded half to even
+// on its exact binary value, as glibc does); the rest go through snprintf.
+int format_g20(double v, char *buf);
+
 // One value per line at 20 significant digits (SaveVectorReal,
-// files.cuh:21-40).  Formatting runs on `threads` host threads (0 = all
+// files.cuh:21-40), as format_g20 writes them.  Formatting runs on `threads` host threads (0 = all
 // hardware threads), the file is written in order.  `path` names the .dat
 // file; the side-car, when selected, replaces its ".dat" by ".bin".
 bool write_real(const std::string &path, const double *v, size_t n, int threads = 0);

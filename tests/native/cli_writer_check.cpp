@@ -1,10 +1,14 @@
 // Sanitizer driver for the CLI's host code (test infrastructure): the .dat
-// writer in every output format, on sizes around its 64 Ki-value blocks, and
+// writer's number formatting against printf, the writer in every output
+// format, on sizes around its 64 Ki-value blocks, and
 // the parameter dump, and the overlapped fetch/write helper, built with -fsanitize=address,undefined by
 // tests/native/Makefile.  Reads every file back and exits 0 when all values
 // round-trip and no sanitizer fired.
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
+#include <cstring>
+#include <random>
 #include <fstream>
 #include <sstream>
 #include <string>
@@ -43,9 +47,58 @@ static bool check(const std::string &dir, const char *fmt, size_t n) {
   return true;
 }
 
+// format_g20 against printf("%.20g"): random bit patterns (every exponent,
+// the snprintf fallback included), random significands at every decimal scale
+// of its exact path and beyond, powers of ten and their neighbours, integers,
+// halves and dyadic fractions (exact ties at the 20th digit)
+static bool check_format() {
+  long bad = 0;
+  auto chk = [&](double v) {
+    char a[64], b[64];
+    opo_cli::format_g20(v, a);
+    std::snprintf(b, sizeof b, "%.20g", v);
+    if (std::strcmp(a, b) != 0 && bad++ < 5) std::printf("format_g20(%a) = %s, printf %s\n", v, a, b);
+  };
+  std::mt19937_64 g(7);
+  for (int i = 0; i < 200000; ++i) {
+    const uint64_t u = g();
+    double v;
+    std::memcpy(&v, &u, sizeof v);
+    chk(v);
+  }
+  std::uniform_real_distribution<double> unit(1.0, 10.0);
+  for (int k = -20; k <= 22; ++k)
+    for (int i = 0; i < 5000; ++i) {
+      const double v = unit(g) * std::pow(10.0, k);
+      chk(v);
+      chk(-v);
+    }
+  for (int k = -20; k <= 22; ++k) {
+    double lo = std::pow(10.0, k), hi = lo;
+    for (int j = 0; j < 4; ++j, lo = std::nextafter(lo, 0.0), hi = std::nextafter(hi, 1e300)) {
+      chk(lo);
+      chk(hi);
+    }
+  }
+  for (int i = 0; i < 100000; ++i) {
+    chk((double)i);
+    chk(i + 0.5);
+    chk((double)(g() >> 11));
+  }
+  for (int j = 0; j < 70; ++j)
+    for (int i = 1; i < 300; ++i) chk(std::ldexp((double)i, -j));
+  for (double v : {0.0, -0.0, 5e-324, 1e-13, 1e20, 99999999999999999999.0, 1.7976931348623157e308}) chk(v);
+  return bad == 0;
+}
+
 int main(int argc, char **argv) {
   const std::string dir = argc > 1 ? argv[1] : ".";
   bool ok = true;
+  {
+    const bool c = check_format();
+    std::printf("format_g20 against printf: %s\n", c ? "ok" : "bad");
+    ok = ok && c;
+  }
   for (const char *fmt : {"dat", "bin", "both"})
     for (size_t n : {(size_t)0, (size_t)1, (size_t)65535, (size_t)65536, (size_t)200001}) {
       const bool c = check(dir, fmt, n);

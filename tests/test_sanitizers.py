@@ -37,7 +37,9 @@ def test_oracle_round_trips_are_sanitizer_clean(built):
 
 
 def test_cli_writer_is_sanitizer_clean(built, tmp_path):
-    # 15 writer cases (3 formats x 5 sizes) and 3 of the overlapped
-    # fetch/write helper (no failure, failure at the first and third item)
+    # the number formatting against printf, 15 writer cases (3 formats x 5
+    # sizes) and 3 of the overlapped fetch/write helper (no failure, failure
+    # at the first and third item)
     out = run_clean([os.path.join(built, "cli_writer_check"), str(tmp_path)])
-    assert out.count(": ok") == 18
+    assert "format_g20 against printf: ok" in out
+    assert out.count(": ok") == 19
