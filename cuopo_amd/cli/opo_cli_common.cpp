@@ -41,23 +41,72 @@ struct Pow5 {
 const Pow5 kPow5;
 const u128 kTen19 = (u128)10000000000000000000ULL, kTen20 = kTen19 * 10;
 
+// 5^q for q = 33..343 as little-endian 64-bit limbs (5^343 < 2^797): the
+// scale factors of magnitudes below 1e-13 down to the smallest normal double
+constexpr int kBigQ = 343, kBigLimbs = 13;
+struct BigPow5 {
+  uint64_t p[kBigQ + 1][kBigLimbs];
+  BigPow5() {
+    uint64_t x[kBigLimbs] = {1};
+    for (int q = 0; q <= kBigQ; ++q) {
+      std::memcpy(p[q], x, sizeof x);
+      u128 carry = 0;
+      for (int i = 0; i < kBigLimbs; ++i) {
+        const u128 y = (u128)x[i] * 5 + carry;
+        x[i] = (uint64_t)y;
+        carry = y >> 64;
+      }
+    }
+  }
+};
+const BigPow5 kBigPow5;
+
 // t = floor(m 2^e 10^q) and whether the exact rest rounds it up (to nearest,
-// ties to even); false when that does not fit the 128-bit arithmetic
+// ties to even); false when that does not fit the integer arithmetic below
 bool scale10(uint64_t m, int e, int q, u128 *t, bool *up) {
-  if (q < 0 || q > 32) return false;
-  const u128 n = (u128)m * kPow5.p[q];
-  const int sh = e + q;  // m 2^e 10^q = n 2^sh
-  if (sh >= 0) {
-    if (sh >= 64 || (n >> (127 - sh)) != 0) return false;
-    *t = n << sh;
-    *up = false;
+  if (q < 0 || q > kBigQ) return false;
+  if (q <= 32) {
+    const u128 n = (u128)m * kPow5.p[q];
+    const int sh = e + q;  // m 2^e 10^q = n 2^sh
+    if (sh >= 0) {
+      if (sh >= 64 || (n >> (127 - sh)) != 0) return false;
+      *t = n << sh;
+      *up = false;
+      return true;
+    }
+    const int s = -sh;
+    if (s >= 127) return false;
+    const u128 r = n & (((u128)1 << s) - 1), half = (u128)1 << (s - 1);
+    *t = n >> s;
+    *up = r > half || (r == half && (*t & 1));
     return true;
   }
-  const int s = -sh;
-  if (s >= 127) return false;
-  const u128 r = n & (((u128)1 << s) - 1), half = (u128)1 << (s - 1);
-  *t = n >> s;
-  *up = r > half || (r == half && (*t & 1));
+  // q > 32: n = m 5^q in limbs, then t = n >> s with s = -(e + q) > 0
+  const int s = -(e + q);
+  if (s <= 0) return false;
+  uint64_t n[kBigLimbs + 1];
+  u128 carry = 0;
+  for (int i = 0; i < kBigLimbs; ++i) {
+    const u128 y = (u128)kBigPow5.p[q][i] * m + carry;
+    n[i] = (uint64_t)y;
+    carry = y >> 64;
+  }
+  n[kBigLimbs] = (uint64_t)carry;
+  constexpr int kBits = 64 * (kBigLimbs + 1);
+  auto word = [&](int pos) -> uint64_t {  // bits pos .. pos+63 of n
+    if (pos >= kBits) return 0;
+    const int li = pos >> 6, bo = pos & 63;
+    uint64_t w = n[li] >> bo;
+    if (bo && li + 1 <= kBigLimbs) w |= n[li + 1] << (64 - bo);
+    return w;
+  };
+  if (s >= kBits || word(s + 128) != 0) return false;  // t would not fit 128 bits
+  *t = (u128)word(s) | ((u128)word(s + 64) << 64);
+  const int hb = s - 1;  // the half bit
+  const bool half = (n[hb >> 6] >> (hb & 63)) & 1;
+  bool below = (n[hb >> 6] & ((1ULL << (hb & 63)) - 1)) != 0;
+  for (int i = 0; i < (hb >> 6) && !below; ++i) below = n[i] != 0;
+  *up = half && (below || (*t & 1));
   return true;
 }
 

@@ -20,9 +20,9 @@ enum class Format { Dat, Bin, Both };
 bool set_format(const std::string &name);
 
 // printf("%.20g") of v into buf (at least 32 bytes, NUL-terminated); returns
-// the length.  Finite values from 1e-13 up to 1e20 in magnitude take an exact
-// integer path (the value's 20 leading decimal digits, rounded half to even
-// on its exact binary value, as glibc does); the rest go through snprintf.
+// the length.  Normal values below 1e20 in magnitude take an exact integer
+// path (the value's 20 leading decimal digits, rounded half to even on its
+// exact binary value, as glibc does); the rest go through snprintf.
 int format_g20(double v, char *buf);
 
 // One value per line at 20 significant digits (SaveVectorReal,

@@ -49,8 +49,9 @@ static bool check(const std::string &dir, const char *fmt, size_t n) {
 
 // format_g20 against printf("%.20g"): random bit patterns (every exponent,
 // the snprintf fallback included), random significands at every decimal scale
-// of its exact path and beyond, powers of ten and their neighbours, integers,
-// halves and dyadic fractions (exact ties at the 20th digit)
+// from the smallest normal double up to beyond its exact path, powers of ten
+// and their neighbours, integers, halves and dyadic fractions (exact ties at
+// the 20th digit)
 static bool check_format() {
   long bad = 0;
   auto chk = [&](double v) {
@@ -73,7 +74,9 @@ static bool check_format() {
       chk(v);
       chk(-v);
     }
-  for (int k = -20; k <= 22; ++k) {
+  for (int k = -308; k < -20; ++k)
+    for (int i = 0; i < 300; ++i) chk(unit(g) * std::pow(10.0, k));
+  for (int k = -308; k <= 22; ++k) {
     double lo = std::pow(10.0, k), hi = lo;
     for (int j = 0; j < 4; ++j, lo = std::nextafter(lo, 0.0), hi = std::nextafter(hi, 1e300)) {
       chk(lo);
